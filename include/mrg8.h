@@ -1,0 +1,195 @@
+/*
+ * mrg8.h -- C ABI of the MI355X-native MRG8 generator (libmrg8_amd.so).
+ *
+ * MRG8 is the 8th-order multiple recursive generator modulo p = 2^31 - 1,
+ *   S[n] = (a0 S[n-1] + a1 S[n-2] + ... + a7 S[n-8]) mod p,
+ * whose 8-word state is kept newest word first (state[0] = S[n-1]).  Every
+ * output of this library is bit-identical to the reference's serial
+ * generator (original/mrg8.cpp) for the same state and position: words are in
+ * [1, p] (a residue of 0 is emitted as p), and doubles are formed by
+ * multiplying with the rounded reciprocal 1.0/2147483647.0.
+ *
+ * The entry points replace the reference's C++ classes `mrg8` (host) and
+ * `mrg8_cuda` (device generation path); each cites the method it stands in
+ * for, as file:line in the reference tree.  The header-only C++ shim
+ * include/mrg8_cuda.hpp rebuilds those class names on top of this ABI.
+ *
+ * Conventions:
+ *   - Every function returns an int status: MRG8_OK (0) or a negative
+ *     MRG8_ERR_* code; mrg8_last_error() gives the calling thread's message.
+ *     Nothing calls exit() (the reference exits on a missing table file).
+ *   - Device pointers are plain addresses on the generator's device; `stream`
+ *     arguments are hipStream_t values passed as void* (NULL = null stream).
+ *   - Generation calls are stream-ordered and do not allocate: the host-side
+ *     state advances immediately by n (as if the call had run), so successive
+ *     calls continue the sequence without a device round trip.
+ *   - Distinct handles may be used from different threads; one handle needs
+ *     external synchronisation.
+ */
+#ifndef MRG8_AMD_H
+#define MRG8_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MRG8_ABI_VERSION 1
+
+/* Status codes. */
+#define MRG8_OK 0
+#define MRG8_ERR_ARG (-1)       /* invalid argument (null pointer, bad format, n out of range) */
+#define MRG8_ERR_HIP (-2)       /* a HIP runtime call failed */
+#define MRG8_ERR_NOMEM (-3)     /* device or host allocation failed */
+#define MRG8_ERR_NODEVICE (-4)  /* no usable gfx950 device / kernels not loadable */
+#define MRG8_ERR_CAPACITY (-5)  /* a fixed capacity (streams, tiles, bytes) would be exceeded */
+#define MRG8_ERR_STATE (-6)     /* call not valid in the object's current state */
+#define MRG8_ERR_NOTFOUND (-7)  /* unknown stream or request id */
+#define MRG8_ERR_IO (-8)        /* file could not be read or written */
+
+/* Output formats. */
+#define MRG8_F64 0     /* double(x) * (1/p), (0, 1]: original/mrg8.cpp:231-239 (mrg8::rand) */
+#define MRG8_F64_01 1  /* double(x - 1) * (1/p), [0, 1): cuda/mrg8_cuda.cu:363 (mrg8_outer_32) */
+#define MRG8_U32 2     /* raw word in [1, p]: original/mrg8.cpp:197-222 (mrg8::randint) */
+
+typedef struct mrg8_gen* mrg8_gen_t;
+typedef struct mrg8_batch* mrg8_batch_t;
+
+/* ---- library ----------------------------------------------------------- */
+
+int mrg8_abi_version(void);
+const char* mrg8_strerror(int status);
+/* Message for the last failing call on this thread ("" if none). */
+const char* mrg8_last_error(void);
+/* Loads the jump tables and kernels on `device` (called implicitly by
+ * mrg8_create on the current device). */
+int mrg8_init_device(int device);
+
+/* ---- generator: construction and state (classes mrg8 / mrg8_cuda) ------- */
+
+/* mrg8_cuda::mrg8_cuda(seed) and mrg8_cuda::mrg8_cuda() (cuda/mrg8_cuda.cu:22-63,
+ * seeding by mcg64ni, original/mrg8.cpp:180-194; seed 0 means 97531).  The
+ * generator is bound to the current HIP device.  The jump table is built in
+ * memory from its closed form; no jump_matrix.txt is read. */
+int mrg8_create(uint32_t seed, mrg8_gen_t* out);
+int mrg8_destroy(mrg8_gen_t g);
+/* mrg8_cuda::seed_init (cuda/mrg8_cuda.cu:66-70). */
+int mrg8_seed_init(mrg8_gen_t g, uint32_t seed);
+/* mrg8::get_state / set_state (original/mrg8.cpp:36-39,56-59).  Unlike
+ * mrg8_cuda (whose set_state leaves the device copy untouched), the state
+ * set here is the one the next device generation starts from. */
+int mrg8_get_state(mrg8_gen_t g, uint32_t state[8]);
+int mrg8_set_state(mrg8_gen_t g, const uint32_t state[8]);
+/* mrg8::jump_ahead(uint64_t) -- mutating form (original/mrg8.cpp:125-129). */
+int mrg8_jump_ahead(mrg8_gen_t g, uint64_t step);
+/* mrg8::jump_ahead(const short[200]) -- 200-bit jump, bits[0] the least
+ * significant bit (original/mrg8.cpp:105-122). */
+int mrg8_jump_ahead_bits(mrg8_gen_t g, const short bits[200]);
+/* mrg8::jump_ahead(uint64_t, uint32_t* new_state) -- non-mutating form of
+ * the CUDA/knl classes (cuda/mrg8.cpp:145-150). */
+int mrg8_jump_state(mrg8_gen_t g, uint64_t step, uint32_t new_state[8]);
+/* mrg8::jump_ahead(const short[200], uint32_t* new_state) (cuda/mrg8.cpp:127-141). */
+int mrg8_jump_state_bits(mrg8_gen_t g, const short bits[200], uint32_t new_state[8]);
+
+/* Pure state functions (no handle, no device): the state after seeding
+ * (original/mrg8.cpp:180-194) and a state advanced by `step` positions
+ * (the new_state form of jump_ahead, cuda/mrg8.cpp:127-150; 200-bit form
+ * with bits[0] least significant). */
+int mrg8_seed_state(uint32_t seed, uint32_t state[8]);
+int mrg8_advance_state(const uint32_t state[8], uint64_t step, uint32_t new_state[8]);
+int mrg8_advance_state_bits(const uint32_t state[8], const short bits[200], uint32_t new_state[8]);
+
+/* ---- generator: device generation (the hot path) ------------------------ */
+
+/* Fill `dptr` (device memory, 8-byte aligned for doubles, 4 for words) with
+ * the next n outputs in `format`, stream-ordered on `stream`; the state
+ * advances by n.  Replaces the rng_outer_32 / rng_inner launches
+ * (cuda/mrg8_cuda.cu:282-399,201-280 via :402-436) with no size limit
+ * (n is 64-bit; the reference takes int and jumps only 32 bits on device). */
+int mrg8_generate(mrg8_gen_t g, void* dptr, uint64_t n, int format, void* stream);
+/* Same, with an explicit launch shape: threads per block (64..1024, multiple
+ * of 64) and outputs per wave (multiple of 64; 0 = automatic).  The output is
+ * identical for every shape. */
+int mrg8_generate_shape(mrg8_gen_t g, void* dptr, uint64_t n, int format, void* stream,
+                        int block_threads, uint64_t outputs_per_wave);
+/* mrg8_cuda::mrg8_outer_32(d_ran, n[, TNUM[, BS]]) (cuda/mrg8_cuda.cu:414-436):
+ * blocking, [0, 1) doubles (MRG8_F64_01).  BS/TNUM keep the reference's
+ * meaning (threads per block / total threads) and pick the launch shape. */
+int mrg8_outer_32(mrg8_gen_t g, double* d_ran, int n, int BS, int TNUM);
+/* mrg8_cuda::mrg8_inner(d_ran, n, TNUM) (cuda/mrg8_cuda.cu:402-412): blocking,
+ * [0, 1) doubles; the same sequence as mrg8_outer_32. */
+int mrg8_inner(mrg8_gen_t g, double* d_ran, int n, int TNUM);
+/* mrg8_cuda::state_cpy_DtH (cuda/mrg8_cuda.cu:72-79).  The host copy of the
+ * state is always current here; this waits for the generator's outstanding
+ * work and returns. */
+int mrg8_state_cpy_DtH(mrg8_gen_t g);
+/* Waits until all generation issued through `g` has finished. */
+int mrg8_synchronize(mrg8_gen_t g);
+
+/* ---- generator: host-memory forms (class mrg8) --------------------------- */
+
+/* mrg8::randint(uint32_t*, int) / mrg8::rand(double*, int)
+ * (original/mrg8.cpp:197-239): generated on the device, copied to the host
+ * buffer; blocking. */
+int mrg8_randint(mrg8_gen_t g, uint32_t* host, uint64_t n);
+int mrg8_rand(mrg8_gen_t g, double* host, uint64_t n);
+/* mrg8::mrg8dnz2(double*, int) (cuda/mrg8.cpp:290-344): the same sequence and
+ * conversion as mrg8_rand. */
+int mrg8_mrg8dnz2(mrg8_gen_t g, double* host, uint64_t n);
+/* mrg8::randint() / mrg8::rand() / operator() -- one value, computed on the
+ * host (original/mrg8.cpp:224-228,241-245). */
+int mrg8_randint1(mrg8_gen_t g, uint32_t* value);
+int mrg8_rand1(mrg8_gen_t g, double* value);
+
+/* ---- jump-table format (jump_matrix.txt) -------------------------------- */
+
+/* Writes A^(2^j) mod p, j = 0..246, in the reference's text layout
+ * (one matrix per line, column-major, original/mrg8.cpp:85-102). */
+int mrg8_write_jump_matrix(const char* path);
+/* Copies the 247*64 table words (column-major per matrix) to `table`. */
+int mrg8_jump_matrix(uint32_t* table, size_t words);
+
+/* ---- batched generation: many streams, one persistent launch ------------ */
+
+/* One stream segment: `count` outputs starting `offset` outputs after the
+ * position whose state is `state`.  `out` is device memory. */
+typedef struct mrg8_stream_desc {
+  uint32_t state[8]; /* newest word first, as mrg8_get_state returns */
+  uint64_t offset;   /* first output index relative to `state` */
+  uint64_t count;    /* outputs to write */
+  void* out;         /* device pointer */
+  int32_t format;    /* MRG8_F64, MRG8_F64_01 or MRG8_U32 */
+  uint32_t reserved; /* must be 0 */
+} mrg8_stream_desc;
+
+/* Launch configuration (picked by mrg8_batch_autotune or set by hand). */
+typedef struct mrg8_batch_config {
+  int32_t block_threads;  /* 64..1024, multiple of 64 */
+  int32_t blocks_per_cu;  /* persistent grid = CUs * blocks_per_cu */
+  uint64_t tile_outputs;  /* outputs per work item, multiple of 64 */
+} mrg8_batch_config;
+
+/* A batch object owns device descriptor and tile tables sized for
+ * `max_streams` segments and `max_tiles` work items, and a HIP graph holding
+ * the persistent launch (no reference counterpart: the reference generates
+ * one stream per blocking call). */
+int mrg8_batch_create(int device, uint32_t max_streams, uint64_t max_tiles, mrg8_batch_t* out);
+int mrg8_batch_destroy(mrg8_batch_t b);
+int mrg8_batch_get_config(mrg8_batch_t b, mrg8_batch_config* cfg);
+int mrg8_batch_set_config(mrg8_batch_t b, const mrg8_batch_config* cfg);
+/* Stages `count` descriptors (host memory) for the next run. */
+int mrg8_batch_set(mrg8_batch_t b, const mrg8_stream_desc* descs, uint32_t count);
+/* Replays the captured launch on `stream` over the staged descriptors. */
+int mrg8_batch_run(mrg8_batch_t b, void* stream);
+/* Times candidate configurations on the staged descriptors (the outputs are
+ * rewritten each trial) and keeps the fastest.  `cache_path` (may be NULL)
+ * is a text file of results keyed by device and workload class. */
+int mrg8_batch_autotune(mrg8_batch_t b, void* stream, const char* cache_path, mrg8_batch_config* best);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MRG8_AMD_H */

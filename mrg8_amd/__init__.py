@@ -1,0 +1,242 @@
+"""MI355X-native MRG8 generator: Python mirror of the reference's C++ classes.
+
+`mrg8_cuda` keeps the method names and argument meaning of the reference's
+``mrg8`` / ``mrg8_cuda`` classes (cuda/mrg8.h:23-79, cuda/mrg8_cuda.h:20-45);
+everything goes through the C ABI of libmrg8_amd.so (include/mrg8.h), whose
+device work runs in hand-written gfx950 kernels.  There is no other backend:
+if the library or a gfx950 device is missing, calls raise.
+
+Formats: F64 = double(x)/p in (0,1] (original/mrg8.cpp ``rand``),
+F64_01 = double(x-1)/p in [0,1) (the CUDA kernels), U32 = raw words in [1, p].
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import capi
+from .capi import F64, F64_01, U32, Mrg8Error  # noqa: F401
+
+P = 2147483647
+
+
+def _u32x8(state) -> "ctypes.Array":
+    a = np.asarray(state, dtype=np.int64)
+    if a.shape != (8,):
+        raise ValueError("state must have 8 words")
+    if (a < 0).any() or (a > 0xFFFFFFFF).any():
+        raise ValueError("state words must be 32-bit unsigned")
+    return (ctypes.c_uint32 * 8)(*[int(x) for x in a])
+
+
+def _bits200(bits) -> "ctypes.Array":
+    arr = (ctypes.c_short * 200)()
+    for b in bits:
+        if not 0 <= int(b) < 200:
+            raise ValueError("bit positions must be in [0, 200)")
+        arr[int(b)] = 1
+    return arr
+
+
+def _ptr(p) -> int:
+    if hasattr(p, "data_ptr"):
+        return int(p.data_ptr())  # torch tensor
+    if hasattr(p, "ptr"):
+        return int(p.ptr)  # mrg8_amd.hip.DeviceBuffer
+    return int(p)
+
+
+def _stream(s):
+    if s is None:
+        return None
+    if hasattr(s, "cuda_stream"):
+        return ctypes.c_void_p(int(s.cuda_stream))  # torch.cuda.Stream
+    if hasattr(s, "handle"):
+        return ctypes.c_void_p(s.handle)
+    return ctypes.c_void_p(int(s))
+
+
+class mrg8_cuda:
+    """One MRG8 stream bound to the current HIP device (mrg8_cuda, cuda/mrg8_cuda.h:20-45)."""
+
+    def __init__(self, seed: int = 0):
+        h = ctypes.c_void_p()
+        capi.call("mrg8_create", ctypes.c_uint32(seed & 0xFFFFFFFF), ctypes.byref(h))
+        self._h = h
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) and self._h.value:
+            capi.lib().mrg8_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):  # pragma: no cover
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- state (mrg8::seed_init / get_state / set_state / jump_ahead) --------
+    def seed_init(self, seed: int) -> None:
+        capi.call("mrg8_seed_init", self._h, ctypes.c_uint32(seed & 0xFFFFFFFF))
+
+    def get_state(self) -> np.ndarray:
+        st = (ctypes.c_uint32 * 8)()
+        capi.call("mrg8_get_state", self._h, st)
+        return np.array(st[:], dtype=np.uint32)
+
+    def set_state(self, state) -> None:
+        capi.call("mrg8_set_state", self._h, _u32x8(state))
+
+    def jump_ahead(self, step: int = 0, bits=None) -> None:
+        """Mutating jump (original/mrg8.cpp:105-129): by `step` or by a 200-bit value given as set bit positions."""
+        if bits is not None:
+            capi.call("mrg8_jump_ahead_bits", self._h, _bits200(bits))
+        else:
+            capi.call("mrg8_jump_ahead", self._h, ctypes.c_uint64(step))
+
+    def jump_state(self, step: int = 0, bits=None) -> np.ndarray:
+        """Non-mutating jump returning the new state (cuda/mrg8.cpp:127-150)."""
+        out = (ctypes.c_uint32 * 8)()
+        if bits is not None:
+            capi.call("mrg8_jump_state_bits", self._h, _bits200(bits), out)
+        else:
+            capi.call("mrg8_jump_state", self._h, ctypes.c_uint64(step), out)
+        return np.array(out[:], dtype=np.uint32)
+
+    # -- device generation ------------------------------------------------------
+    def generate(self, dptr, n: int, fmt: int = F64, stream=None, block_threads: int = 0, tile: int = 0) -> None:
+        """Stream-ordered fill of n outputs at device address `dptr`; the state advances by n."""
+        if block_threads or tile:
+            capi.call("mrg8_generate_shape", self._h, ctypes.c_void_p(_ptr(dptr)), ctypes.c_uint64(n), fmt,
+                      _stream(stream), block_threads, ctypes.c_uint64(tile))
+        else:
+            capi.call("mrg8_generate", self._h, ctypes.c_void_p(_ptr(dptr)), ctypes.c_uint64(n), fmt,
+                      _stream(stream))
+
+    def mrg8_outer_32(self, d_ran, n: int, BS: int = 64, TNUM: int = 64 * 1024) -> None:
+        """cuda/mrg8_cuda.cu:414-436: blocking, [0,1) doubles."""
+        capi.call("mrg8_outer_32", self._h, ctypes.c_void_p(_ptr(d_ran)), n, BS, TNUM)
+
+    def mrg8_inner(self, d_ran, n: int, TNUM: int = 64 * 1024) -> None:
+        """cuda/mrg8_cuda.cu:402-412: blocking, [0,1) doubles."""
+        capi.call("mrg8_inner", self._h, ctypes.c_void_p(_ptr(d_ran)), n, TNUM)
+
+    def state_cpy_DtH(self) -> None:
+        capi.call("mrg8_state_cpy_DtH", self._h)
+
+    def synchronize(self) -> None:
+        capi.call("mrg8_synchronize", self._h)
+
+    # -- host-memory forms (class mrg8) --------------------------------------------
+    def randint(self, n: int | None = None):
+        if n is None:
+            v = ctypes.c_uint32()
+            capi.call("mrg8_randint1", self._h, ctypes.byref(v))
+            return int(v.value)
+        out = np.empty(n, dtype=np.uint32)
+        capi.call("mrg8_randint", self._h, ctypes.c_void_p(out.ctypes.data), ctypes.c_uint64(n))
+        return out
+
+    def rand(self, n: int | None = None):
+        if n is None:
+            v = ctypes.c_double()
+            capi.call("mrg8_rand1", self._h, ctypes.byref(v))
+            return float(v.value)
+        out = np.empty(n, dtype=np.float64)
+        capi.call("mrg8_rand", self._h, ctypes.c_void_p(out.ctypes.data), ctypes.c_uint64(n))
+        return out
+
+    def __call__(self) -> float:
+        return self.rand()
+
+    def mrg8dnz2(self, n: int) -> np.ndarray:
+        out = np.empty(n, dtype=np.float64)
+        capi.call("mrg8_mrg8dnz2", self._h, ctypes.c_void_p(out.ctypes.data), ctypes.c_uint64(n))
+        return out
+
+
+# The host class of the reference has the same surface minus the device launchers.
+mrg8 = mrg8_cuda
+
+
+def seed_state(seed: int) -> np.ndarray:
+    """State after seeding (original/mrg8.cpp:180-194)."""
+    st = (ctypes.c_uint32 * 8)()
+    capi.call("mrg8_seed_state", ctypes.c_uint32(seed & 0xFFFFFFFF), st)
+    return np.array(st[:], dtype=np.uint32)
+
+
+def advance_state(state, step: int = 0, bits=None) -> np.ndarray:
+    """`state` advanced by `step` positions (or by a 200-bit value given as set bits)."""
+    out = (ctypes.c_uint32 * 8)()
+    if bits is not None:
+        capi.call("mrg8_advance_state_bits", _u32x8(state), _bits200(bits), out)
+    else:
+        capi.call("mrg8_advance_state", _u32x8(state), ctypes.c_uint64(step), out)
+    return np.array(out[:], dtype=np.uint32)
+
+
+def jump_matrix() -> np.ndarray:
+    """A^(2^j) mod p, j = 0..246, column-major per matrix (jump_matrix.txt layout)."""
+    t = np.zeros(247 * 64, dtype=np.uint32)
+    capi.call("mrg8_jump_matrix", t.ctypes.data_as(capi.u32p), ctypes.c_size_t(t.size))
+    return t
+
+
+def write_jump_matrix(path: str) -> None:
+    capi.call("mrg8_write_jump_matrix", path.encode())
+
+
+class Batch:
+    """Many stream segments generated by one persistent, graph-captured launch."""
+
+    def __init__(self, max_streams: int, max_tiles: int = 1 << 24, device: int = 0):
+        h = ctypes.c_void_p()
+        capi.call("mrg8_batch_create", device, ctypes.c_uint32(max_streams), ctypes.c_uint64(max_tiles),
+                  ctypes.byref(h))
+        self._h = h
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) and self._h.value:
+            capi.lib().mrg8_batch_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):  # pragma: no cover
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def config(self):
+        c = capi.BatchConfig()
+        capi.call("mrg8_batch_get_config", self._h, ctypes.byref(c))
+        return {"block_threads": c.block_threads, "blocks_per_cu": c.blocks_per_cu, "tile_outputs": c.tile_outputs}
+
+    def set_config(self, block_threads: int, blocks_per_cu: int, tile_outputs: int) -> None:
+        c = capi.BatchConfig(block_threads, blocks_per_cu, tile_outputs)
+        capi.call("mrg8_batch_set_config", self._h, ctypes.byref(c))
+
+    def set(self, segments) -> None:
+        """segments: iterable of (state[8], offset, count, device_ptr, fmt)."""
+        segs = list(segments)
+        arr = (capi.StreamDesc * max(1, len(segs)))()
+        for i, (st, off, cnt, ptr, fmt) in enumerate(segs):
+            d = arr[i]
+            d.state = _u32x8(st)
+            d.offset = int(off)
+            d.count = int(cnt)
+            d.out = _ptr(ptr)
+            d.format = int(fmt)
+            d.reserved = 0
+        capi.call("mrg8_batch_set", self._h, arr, ctypes.c_uint32(len(segs)))
+
+    def run(self, stream=None) -> None:
+        capi.call("mrg8_batch_run", self._h, _stream(stream))
+
+    def autotune(self, stream=None, cache_path: str | None = None):
+        c = capi.BatchConfig()
+        capi.call("mrg8_batch_autotune", self._h, _stream(stream), cache_path.encode() if cache_path else None,
+                  ctypes.byref(c))
+        return {"block_threads": c.block_threads, "blocks_per_cu": c.blocks_per_cu, "tile_outputs": c.tile_outputs}

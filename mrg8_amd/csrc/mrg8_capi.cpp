@@ -1,0 +1,510 @@
+// C ABI of the MI355X-native MRG8 generator: generator handles, device
+// generation, host-memory forms, jump-table output.  Declared in include/mrg8.h.
+#include "../../include/mrg8.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "mrg8_device.hpp"
+#include "mrg8_internal.hpp"
+
+namespace mrg8 {
+
+// ---- errors ----------------------------------------------------------------
+
+namespace {
+thread_local std::string g_last_error;
+}
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return code;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+  return fail(e == hipErrorOutOfMemory ? MRG8_ERR_NOMEM : MRG8_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+void clear_error() { g_last_error.clear(); }
+
+// ---- host tables -------------------------------------------------------------
+
+const JumpTable& host_jump_table() {
+  static const JumpTable t;
+  return t;
+}
+
+// ---- per-device state ------------------------------------------------------------
+
+namespace {
+std::mutex g_dev_mu;
+std::vector<std::unique_ptr<DeviceContext>> g_devices;
+}  // namespace
+
+int device_context(int device, DeviceContext** out) {
+  std::lock_guard<std::mutex> lk(g_dev_mu);
+  int count = 0;
+  hipError_t e = hipGetDeviceCount(&count);
+  if (e != hipSuccess || count <= 0) return fail(MRG8_ERR_NODEVICE, "no HIP device available");
+  if (device < 0 || device >= count) return fail(MRG8_ERR_ARG, "device %d out of range (have %d)", device, count);
+  if ((int)g_devices.size() < count) g_devices.resize(count);
+  if (g_devices[device]) {
+    *out = g_devices[device].get();
+    return MRG8_OK;
+  }
+  hipDeviceProp_t prop;
+  if ((e = hipGetDeviceProperties(&prop, device)) != hipSuccess) return hip_fail(e, "hipGetDeviceProperties");
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    return fail(MRG8_ERR_NODEVICE, "device %d is %s; this library is built for gfx950 only", device, prop.gcnArchName);
+
+  int prev = 0;
+  (void)hipGetDevice(&prev);
+  if ((e = hipSetDevice(device)) != hipSuccess) return hip_fail(e, "hipSetDevice");
+
+  // Coefficient records: block rows (row 0 of A^(k+1)) and jump rows.
+  std::vector<CoefRecord> lane(kLaneRows), jump((size_t)kJumpBits * 8);
+  const std::vector<uint32_t> rows = block_rows(kLaneRows);
+  for (int k = 0; k < kLaneRows; ++k) {
+    if (!all_nonzero(&rows[(size_t)k * 8])) {
+      (void)hipSetDevice(prev);
+      return fail(MRG8_ERR_STATE, "block row %d has a zero entry", k);
+    }
+    lane[k] = make_record(&rows[(size_t)k * 8]);
+  }
+  const JumpTable& jt = host_jump_table();
+  for (int b = 0; b < kJumpBits; ++b)
+    for (int r = 0; r < 8; ++r) {
+      const uint32_t* row = jt.at(b) + r * 8;
+      // Rows are either fully non-zero or (for A, A^2, A^4) a shifted unit
+      // vector; both are handled exactly by the support mask.
+      jump[(size_t)b * 8 + r] = make_record(row);
+    }
+
+  auto ctx = std::make_unique<DeviceContext>();
+  ctx->device = device;
+  ctx->num_cu = prop.multiProcessorCount;
+  void* p = nullptr;
+  const size_t bytes = sizeof(CoefRecord) * (lane.size() + jump.size());
+  if ((e = hipMalloc(&p, bytes)) != hipSuccess) {
+    (void)hipSetDevice(prev);
+    return hip_fail(e, "hipMalloc(tables)");
+  }
+  ctx->tables.lane = static_cast<CoefRecord*>(p);
+  ctx->tables.jump = ctx->tables.lane + lane.size();
+  e = hipMemcpy(ctx->tables.lane, lane.data(), sizeof(CoefRecord) * lane.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(ctx->tables.jump, jump.data(), sizeof(CoefRecord) * jump.size(), hipMemcpyHostToDevice);
+  (void)hipSetDevice(prev);
+  if (e != hipSuccess) {
+    (void)hipFree(p);
+    return hip_fail(e, "hipMemcpy(tables)");
+  }
+  *out = ctx.get();
+  g_devices[device] = std::move(ctx);
+  return MRG8_OK;
+}
+
+bool state_valid(const uint32_t st[8]) {
+  for (int i = 0; i < 8; ++i)
+    if (st[i] > kP) return false;
+  return true;
+}
+
+int format_size(int format) {
+  switch (format) {
+    case MRG8_F64:
+    case MRG8_F64_01: return 8;
+    case MRG8_U32: return 4;
+    default: return 0;
+  }
+}
+
+// Outputs per lane for a destination address: 16-byte stores when aligned.
+int pick_per_lane(int format, const void* p) {
+  const uintptr_t a = reinterpret_cast<uintptr_t>(p);
+  if (format == MRG8_U32) return (a & 15u) == 0 ? 4 : ((a & 7u) == 0 ? 2 : 1);
+  return (a & 15u) == 0 ? 2 : 1;
+}
+
+// Default outputs per wave for a single-stream fill of n outputs.
+uint64_t default_tile(const DeviceContext& ctx, uint64_t n, int per_lane) {
+  const uint64_t quantum = 64ull * per_lane;
+  const uint64_t target_waves = (uint64_t)ctx.num_cu * 32;  // 8 waves per SIMD
+  uint64_t tile = (n + target_waves - 1) / target_waves;
+  if (tile < 16 * quantum) tile = 16 * quantum;             // keep the jump prologue amortised
+  return (tile + quantum - 1) / quantum * quantum;
+}
+
+int fill_device(DeviceContext& ctx, const uint32_t state[8], uint64_t offset, void* dptr, uint64_t n, int format,
+                hipStream_t stream, int block_threads, uint64_t tile) {
+  if (n == 0) return MRG8_OK;
+  const int pl = pick_per_lane(format, dptr);
+  if (block_threads <= 0) block_threads = 256;
+  if (block_threads < 64 || block_threads > 1024 || block_threads % 64)
+    return fail(MRG8_ERR_ARG, "block_threads %d must be a multiple of 64 in [64, 1024]", block_threads);
+  const uint64_t quantum = 64ull * pl;
+  if (tile == 0) tile = default_tile(ctx, n, pl);
+  if (tile % quantum) tile = (tile + quantum - 1) / quantum * quantum;
+  FillArgs a{};
+  a.out = dptr;
+  a.n = n;
+  a.tile = tile;
+  std::memcpy(a.start.state, state, sizeof a.start.state);
+  a.start.offset = offset;
+  a.lane = ctx.tables.lane;
+  a.jump = ctx.tables.jump;
+  int prev = 0;
+  (void)hipGetDevice(&prev);
+  if (prev != ctx.device) (void)hipSetDevice(ctx.device);
+  hipError_t e = launch_fill(a, format, pl, block_threads, stream);
+  if (prev != ctx.device) (void)hipSetDevice(prev);
+  if (e != hipSuccess) return hip_fail(e, "fill launch");
+  return MRG8_OK;
+}
+
+}  // namespace mrg8
+
+using namespace mrg8;
+
+struct mrg8_gen {
+  DeviceContext* ctx = nullptr;
+  uint32_t state[8] = {0};
+  uint32_t seed = 0;
+  hipStream_t last_stream = nullptr;
+  bool issued = false;
+  void* staging = nullptr;  // device buffer for the host-memory forms
+  size_t staging_bytes = 0;
+};
+
+// ---- library ----------------------------------------------------------------
+
+extern "C" int mrg8_abi_version(void) { return MRG8_ABI_VERSION; }
+
+extern "C" const char* mrg8_strerror(int status) {
+  switch (status) {
+    case MRG8_OK: return "ok";
+    case MRG8_ERR_ARG: return "invalid argument";
+    case MRG8_ERR_HIP: return "HIP runtime error";
+    case MRG8_ERR_NOMEM: return "out of memory";
+    case MRG8_ERR_NODEVICE: return "no usable gfx950 device";
+    case MRG8_ERR_CAPACITY: return "capacity exceeded";
+    case MRG8_ERR_STATE: return "invalid state for this call";
+    case MRG8_ERR_NOTFOUND: return "not found";
+    case MRG8_ERR_IO: return "I/O error";
+    default: return "unknown status";
+  }
+}
+
+extern "C" const char* mrg8_last_error(void) { return g_last_error.c_str(); }
+
+extern "C" int mrg8_init_device(int device) {
+  clear_error();
+  DeviceContext* ctx = nullptr;
+  return device_context(device, &ctx);
+}
+
+// ---- generator ----------------------------------------------------------------
+
+extern "C" int mrg8_create(uint32_t seed, mrg8_gen_t* out) {
+  clear_error();
+  if (!out) return fail(MRG8_ERR_ARG, "null output handle");
+  *out = nullptr;
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return fail(MRG8_ERR_NODEVICE, "hipGetDevice: %s", hipGetErrorString(e));
+  DeviceContext* ctx = nullptr;
+  int rc = device_context(dev, &ctx);
+  if (rc) return rc;
+  mrg8_gen* g = new (std::nothrow) mrg8_gen;
+  if (!g) return fail(MRG8_ERR_NOMEM, "host allocation failed");
+  g->ctx = ctx;
+  g->seed = seed;
+  seed_state(seed, g->state);
+  *out = g;
+  return MRG8_OK;
+}
+
+extern "C" int mrg8_destroy(mrg8_gen_t g) {
+  clear_error();
+  if (!g) return MRG8_OK;
+  if (g->issued) {
+    (void)hipSetDevice(g->ctx->device);
+    (void)hipStreamSynchronize(g->last_stream);
+  }
+  if (g->staging) (void)hipFree(g->staging);
+  delete g;
+  return MRG8_OK;
+}
+
+extern "C" int mrg8_seed_init(mrg8_gen_t g, uint32_t seed) {
+  clear_error();
+  if (!g) return fail(MRG8_ERR_ARG, "null generator");
+  g->seed = seed;
+  seed_state(seed, g->state);
+  return MRG8_OK;
+}
+
+extern "C" int mrg8_get_state(mrg8_gen_t g, uint32_t state[8]) {
+  clear_error();
+  if (!g || !state) return fail(MRG8_ERR_ARG, "null argument");
+  std::memcpy(state, g->state, sizeof g->state);
+  return MRG8_OK;
+}
+
+extern "C" int mrg8_set_state(mrg8_gen_t g, const uint32_t state[8]) {
+  clear_error();
+  if (!g || !state) return fail(MRG8_ERR_ARG, "null argument");
+  if (!state_valid(state)) return fail(MRG8_ERR_ARG, "state words must be at most 2^31 - 1");
+  std::memcpy(g->state, state, sizeof g->state);
+  return MRG8_OK;
+}
+
+extern "C" int mrg8_jump_ahead(mrg8_gen_t g, uint64_t step) {
+  clear_error();
+  if (!g) return fail(MRG8_ERR_ARG, "null generator");
+  jump64(host_jump_table(), step, g->state);
+  return MRG8_OK;
+}
+
+extern "C" int mrg8_jump_ahead_bits(mrg8_gen_t g, const short bits[200]) {
+  clear_error();
+  if (!g || !bits) return fail(MRG8_ERR_ARG, "null argument");
+  jump200(host_jump_table(), bits, g->state);
+  return MRG8_OK;
+}
+
+extern "C" int mrg8_jump_state(mrg8_gen_t g, uint64_t step, uint32_t new_state[8]) {
+  clear_error();
+  if (!g || !new_state) return fail(MRG8_ERR_ARG, "null argument");
+  uint32_t st[8];
+  std::memcpy(st, g->state, sizeof st);
+  jump64(host_jump_table(), step, st);
+  std::memcpy(new_state, st, sizeof st);
+  return MRG8_OK;
+}
+
+extern "C" int mrg8_jump_state_bits(mrg8_gen_t g, const short bits[200], uint32_t new_state[8]) {
+  clear_error();
+  if (!g || !bits || !new_state) return fail(MRG8_ERR_ARG, "null argument");
+  uint32_t st[8];
+  std::memcpy(st, g->state, sizeof st);
+  jump200(host_jump_table(), bits, st);
+  std::memcpy(new_state, st, sizeof st);
+  return MRG8_OK;
+}
+
+extern "C" int mrg8_seed_state(uint32_t seed, uint32_t state[8]) {
+  clear_error();
+  if (!state) return fail(MRG8_ERR_ARG, "null state");
+  seed_state(seed, state);
+  return MRG8_OK;
+}
+
+extern "C" int mrg8_advance_state(const uint32_t state[8], uint64_t step, uint32_t new_state[8]) {
+  clear_error();
+  if (!state || !new_state) return fail(MRG8_ERR_ARG, "null argument");
+  if (!state_valid(state)) return fail(MRG8_ERR_ARG, "state words must be at most 2^31 - 1");
+  uint32_t st[8];
+  std::memcpy(st, state, sizeof st);
+  jump64(host_jump_table(), step, st);
+  std::memcpy(new_state, st, sizeof st);
+  return MRG8_OK;
+}
+
+extern "C" int mrg8_advance_state_bits(const uint32_t state[8], const short bits[200], uint32_t new_state[8]) {
+  clear_error();
+  if (!state || !bits || !new_state) return fail(MRG8_ERR_ARG, "null argument");
+  if (!state_valid(state)) return fail(MRG8_ERR_ARG, "state words must be at most 2^31 - 1");
+  uint32_t st[8];
+  std::memcpy(st, state, sizeof st);
+  jump200(host_jump_table(), bits, st);
+  std::memcpy(new_state, st, sizeof st);
+  return MRG8_OK;
+}
+
+static int generate_impl(mrg8_gen_t g, void* dptr, uint64_t n, int format, void* stream, int block_threads,
+                         uint64_t tile) {
+  if (!g) return fail(MRG8_ERR_ARG, "null generator");
+  const int esz = format_size(format);
+  if (!esz) return fail(MRG8_ERR_ARG, "unknown format %d", format);
+  if (n == 0) return MRG8_OK;
+  if (!dptr) return fail(MRG8_ERR_ARG, "null output pointer");
+  if (reinterpret_cast<uintptr_t>(dptr) % (uintptr_t)esz)
+    return fail(MRG8_ERR_ARG, "output pointer must be %d-byte aligned", esz);
+  if (n > (UINT64_MAX / 8)) return fail(MRG8_ERR_ARG, "n too large");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  int rc = fill_device(*g->ctx, g->state, 0, dptr, n, format, s, block_threads, tile);
+  if (rc) return rc;
+  g->last_stream = s;
+  g->issued = true;
+  jump64(host_jump_table(), n, g->state);  // the state after n outputs
+  return MRG8_OK;
+}
+
+extern "C" int mrg8_generate(mrg8_gen_t g, void* dptr, uint64_t n, int format, void* stream) {
+  clear_error();
+  return generate_impl(g, dptr, n, format, stream, 0, 0);
+}
+
+extern "C" int mrg8_generate_shape(mrg8_gen_t g, void* dptr, uint64_t n, int format, void* stream,
+                                   int block_threads, uint64_t outputs_per_wave) {
+  clear_error();
+  return generate_impl(g, dptr, n, format, stream, block_threads, outputs_per_wave);
+}
+
+extern "C" int mrg8_synchronize(mrg8_gen_t g) {
+  clear_error();
+  if (!g) return fail(MRG8_ERR_ARG, "null generator");
+  if (!g->issued) return MRG8_OK;
+  int prev = 0;
+  (void)hipGetDevice(&prev);
+  (void)hipSetDevice(g->ctx->device);
+  hipError_t e = hipStreamSynchronize(g->last_stream);
+  (void)hipSetDevice(prev);
+  if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+  return MRG8_OK;
+}
+
+// The reference's launchers take int n, BS (threads per block) and TNUM
+// (total threads); they block until the kernel is done.
+static int drop_in_fill(mrg8_gen_t g, double* d, int n, int bs, int tnum) {
+  if (!g) return fail(MRG8_ERR_ARG, "null generator");
+  if (n < 0) return fail(MRG8_ERR_ARG, "n must be non-negative");
+  if (bs <= 0 || tnum <= 0) return fail(MRG8_ERR_ARG, "BS and TNUM must be positive");
+  if (n == 0) return MRG8_OK;
+  // TNUM threads in waves of 64: each wave covers n / (TNUM / 64) outputs.
+  const uint64_t waves = std::max<uint64_t>(1, (uint64_t)tnum / 64);
+  const int pl = pick_per_lane(MRG8_F64_01, d);
+  const uint64_t quantum = 64ull * pl;
+  uint64_t tile = ((uint64_t)n + waves - 1) / waves;
+  tile = std::max<uint64_t>(quantum, (tile + quantum - 1) / quantum * quantum);
+  int block = bs < 64 ? 64 : (bs > 1024 ? 1024 : bs / 64 * 64);
+  int rc = generate_impl(g, d, (uint64_t)n, MRG8_F64_01, nullptr, block, tile);
+  if (rc) return rc;
+  return mrg8_synchronize(g);
+}
+
+extern "C" int mrg8_outer_32(mrg8_gen_t g, double* d_ran, int n, int BS, int TNUM) {
+  clear_error();
+  return drop_in_fill(g, d_ran, n, BS, TNUM);
+}
+
+extern "C" int mrg8_inner(mrg8_gen_t g, double* d_ran, int n, int TNUM) {
+  clear_error();
+  return drop_in_fill(g, d_ran, n, 64, TNUM);
+}
+
+extern "C" int mrg8_state_cpy_DtH(mrg8_gen_t g) {
+  clear_error();
+  return mrg8_synchronize(g);
+}
+
+// ---- host-memory forms ------------------------------------------------------------
+
+static int host_fill(mrg8_gen_t g, void* host, uint64_t n, int format) {
+  if (!g) return fail(MRG8_ERR_ARG, "null generator");
+  if (n == 0) return MRG8_OK;
+  if (!host) return fail(MRG8_ERR_ARG, "null output pointer");
+  const size_t esz = (size_t)format_size(format);
+  const size_t chunk_bytes = (size_t)64 << 20;
+  int prev = 0;
+  (void)hipGetDevice(&prev);
+  (void)hipSetDevice(g->ctx->device);
+  hipError_t e = hipSuccess;
+  if (!g->staging) {
+    e = hipMalloc(&g->staging, chunk_bytes);
+    if (e != hipSuccess) {
+      g->staging = nullptr;
+      (void)hipSetDevice(prev);
+      return hip_fail(e, "hipMalloc(staging)");
+    }
+    g->staging_bytes = chunk_bytes;
+  }
+  // Finish earlier stream-ordered work on this generator first.
+  if (g->issued) (void)hipStreamSynchronize(g->last_stream);
+  const uint64_t per = chunk_bytes / esz;
+  int rc = MRG8_OK;
+  for (uint64_t done = 0; done < n && rc == MRG8_OK;) {
+    const uint64_t m = std::min<uint64_t>(per, n - done);
+    rc = fill_device(*g->ctx, g->state, 0, g->staging, m, format, nullptr, 0, 0);
+    if (rc) break;
+    jump64(host_jump_table(), m, g->state);
+    e = hipMemcpy(static_cast<char*>(host) + done * esz, g->staging, m * esz, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) rc = hip_fail(e, "hipMemcpy(D2H)");
+    done += m;
+  }
+  (void)hipSetDevice(prev);
+  return rc;
+}
+
+extern "C" int mrg8_randint(mrg8_gen_t g, uint32_t* host, uint64_t n) {
+  clear_error();
+  return host_fill(g, host, n, MRG8_U32);
+}
+
+extern "C" int mrg8_rand(mrg8_gen_t g, double* host, uint64_t n) {
+  clear_error();
+  return host_fill(g, host, n, MRG8_F64);
+}
+
+extern "C" int mrg8_mrg8dnz2(mrg8_gen_t g, double* host, uint64_t n) {
+  clear_error();
+  // cuda/mrg8.cpp:290-339 produces the same words and conversion as rand():
+  // its 1024-word blocks only change the order of evaluation.
+  return host_fill(g, host, n, MRG8_F64);
+}
+
+extern "C" int mrg8_randint1(mrg8_gen_t g, uint32_t* value) {
+  clear_error();
+  if (!g || !value) return fail(MRG8_ERR_ARG, "null argument");
+  *value = step1(g->state);
+  return MRG8_OK;
+}
+
+extern "C" int mrg8_rand1(mrg8_gen_t g, double* value) {
+  clear_error();
+  if (!g || !value) return fail(MRG8_ERR_ARG, "null argument");
+  const double rnorm = 1.0 / 2147483647.0;
+  *value = (double)step1(g->state) * rnorm;
+  return MRG8_OK;
+}
+
+// ---- jump table output -----------------------------------------------------------
+
+extern "C" int mrg8_jump_matrix(uint32_t* table, size_t words) {
+  clear_error();
+  if (!table) return fail(MRG8_ERR_ARG, "null table");
+  if (words < (size_t)kNumJump * 64) return fail(MRG8_ERR_ARG, "table needs %d words", kNumJump * 64);
+  const JumpTable& t = host_jump_table();
+  for (int j = 0; j < kNumJump; ++j)
+    for (int r = 0; r < 8; ++r)
+      for (int c = 0; c < 8; ++c) table[(size_t)j * 64 + r + 8 * c] = t.at(j)[r * 8 + c];  // column-major
+  return MRG8_OK;
+}
+
+extern "C" int mrg8_write_jump_matrix(const char* path) {
+  clear_error();
+  if (!path) return fail(MRG8_ERR_ARG, "null path");
+  std::vector<uint32_t> tab((size_t)kNumJump * 64);
+  mrg8_jump_matrix(tab.data(), tab.size());
+  FILE* f = std::fopen(path, "w");
+  if (!f) return fail(MRG8_ERR_IO, "cannot open %s", path);
+  for (int j = 0; j < kNumJump; ++j) {
+    for (int k = 0; k < 64; ++k) std::fprintf(f, "%10u  ", tab[(size_t)j * 64 + k]);
+    std::fputc('\n', f);
+  }
+  if (std::fclose(f) != 0) return fail(MRG8_ERR_IO, "cannot write %s", path);
+  return MRG8_OK;
+}

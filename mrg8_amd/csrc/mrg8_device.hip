@@ -1,0 +1,288 @@
+// gfx950 kernels for MRG8 generation.
+//
+// Replaces the CUDA generation kernels rng_outer_32 and rng_inner
+// (cuda/mrg8_cuda.cu:201-399).  Design:
+//   * A wave owns a contiguous run of outputs of one stream.  The 8-word
+//     state is wave-uniform (scalar registers); lane k holds the centred
+//     coefficients of row 0 of A^(k+1) (one record per output it makes), so
+//     one iteration yields 64 * PL consecutive outputs with eight signed
+//     32x32->64 multiply-adds each (v_mad_i64_i32) and a Mersenne fold.
+//   * The next state is the last eight outputs, read back with v_readlane;
+//     no LDS and no barrier in the loop.
+//   * Outputs leave through coalesced non-temporal vector stores (8 or 16
+//     bytes per lane).
+//   * A wave reaches its first position with a jump: the state is multiplied
+//     by A^(2^b) for every set bit b of the 64-bit offset.  Eight lanes per
+//     matrix compute one row each; eight matrices are fetched per round so a
+//     round costs one memory latency.
+#include "mrg8_device.hpp"
+
+namespace mrg8 {
+namespace {
+
+constexpr double kRnorm = 1.0 / 2147483647.0;  // original/mrg8.cpp:233
+
+typedef double d2_t __attribute__((ext_vector_type(2)));
+typedef uint32_t u2_t __attribute__((ext_vector_type(2)));
+typedef uint32_t u4_t __attribute__((ext_vector_type(4)));
+
+struct Coef {
+  int32_t c[8];
+  uint64_t b0;
+  uint32_t support;
+};
+
+__device__ __forceinline__ Coef load_coef(const CoefRecord* __restrict__ p) {
+  const u4_t* q = reinterpret_cast<const u4_t*>(p);
+  u4_t a = q[0], b = q[1], c = q[2];
+  Coef k;
+  k.c[0] = (int32_t)a.x; k.c[1] = (int32_t)a.y; k.c[2] = (int32_t)a.z; k.c[3] = (int32_t)a.w;
+  k.c[4] = (int32_t)b.x; k.c[5] = (int32_t)b.y; k.c[6] = (int32_t)b.z; k.c[7] = (int32_t)b.w;
+  k.b0 = (uint64_t)c.x | ((uint64_t)c.y << 32);
+  k.support = c.z;
+  return k;
+}
+
+// w = 2 * hi + lo as one 32x32->64 multiply-add (2^32 == 2 mod p).
+__device__ __forceinline__ uint64_t fold_hi(uint32_t hi, uint32_t lo) {
+  uint64_t w, carry;
+  asm("v_mad_u64_u32 %0, %1, %2, 2, %3" : "=v"(w), "=s"(carry) : "v"(hi), "v"((uint64_t)lo));
+  return w;
+}
+
+// Canonical residue r in [0, p) of V = B0 + sum_j c_j e_j, where e_j = s_j - 2^30;
+// the reference's word for the dot product is r + 1 (see CoefRecord).
+__device__ __forceinline__ uint32_t dot_residue(const Coef& k, const int32_t e[8]) {
+  uint64_t v = k.b0;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) v += (uint64_t)((int64_t)k.c[j] * (int64_t)e[j]);
+  const uint64_t w = fold_hi((uint32_t)(v >> 32), (uint32_t)v);        // < 3 * 2^32
+  const uint32_t x = ((uint32_t)w & kP) + (uint32_t)(w >> 31);          // < 2^31 + 6
+  return min(x, x - kP);  // x - p wraps above x when x < p
+}
+
+__device__ __forceinline__ void centre(const uint32_t s[8], int32_t e[8]) {
+#pragma unroll
+  for (int j = 0; j < 8; ++j) e[j] = (int32_t)(s[j] - kCenter);
+}
+
+// Advances the wave-uniform state by `off` positions with the reference's
+// exact semantics (words of A^(2^b) s, a residue of 0 written as p unless the
+// exact sum is 0).
+__device__ void jump_uniform(uint32_t s[8], uint64_t off, const CoefRecord* __restrict__ jrec) {
+  const int lane = threadIdx.x & 63;
+  const int grp = lane >> 3, row = lane & 7;
+  while (off) {
+    uint64_t x = off;
+    for (int i = 0; i < grp; ++i) x &= x - 1;  // this group's set bit
+    const int nb = min(__popcll(off), 8);
+    Coef k = {};
+    if (x) k = load_coef(jrec + (__builtin_ctzll(x) * 8 + row));
+    for (int t = 0; t < nb; ++t) {
+      int32_t e[8];
+      uint32_t nz = 0;
+      centre(s, e);
+#pragma unroll
+      for (int j = 0; j < 8; ++j) nz |= (s[j] != 0u ? 1u : 0u) << j;
+      const uint32_t y = (k.support & nz) ? dot_residue(k, e) + 1u : 0u;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) s[j] = __builtin_amdgcn_readlane(y, 8 * t + j);
+    }
+    for (int t = 0; t < nb; ++t) off &= off - 1;
+  }
+}
+
+template <int FMT>
+struct Out;
+template <>
+struct Out<0> { typedef double T; };
+template <>
+struct Out<1> { typedef double T; };
+template <>
+struct Out<2> { typedef uint32_t T; };
+
+// Output value for residue r, the reference's word being x = r + 1.
+template <int FMT>
+__device__ __forceinline__ typename Out<FMT>::T convert(uint32_t r) {
+  if constexpr (FMT == 0) return __fma_rn((double)r, kRnorm, kRnorm);  // double(x) * (1/p), one rounding: original/mrg8.cpp:236
+  else if constexpr (FMT == 1) return (double)r * kRnorm;              // double(x - 1) * (1/p): cuda/mrg8_cuda.cu:363
+  else return r + 1u;                                                  // x: original/mrg8.cpp:219
+}
+
+// Output value of the all-zero state (whose words stay 0).
+template <int FMT>
+__device__ __forceinline__ typename Out<FMT>::T convert_zero() {
+  if constexpr (FMT == 0) return 0.0;
+  else if constexpr (FMT == 1) return (double)0xffffffffu * kRnorm;  // (x - 1) wraps in 32 bits
+  else return 0u;
+}
+
+// Non-temporal store of PL consecutive outputs of one lane.
+template <int FMT, int PL>
+__device__ __forceinline__ void store_lane(typename Out<FMT>::T* p, const uint32_t r[PL]) {
+  if constexpr (FMT == 2 && PL == 4) {
+    u4_t v = {r[0] + 1u, r[1] + 1u, r[2] + 1u, r[3] + 1u};
+    __builtin_nontemporal_store(v, reinterpret_cast<u4_t*>(p));
+  } else if constexpr (FMT == 2 && PL == 2) {
+    u2_t v = {r[0] + 1u, r[1] + 1u};
+    __builtin_nontemporal_store(v, reinterpret_cast<u2_t*>(p));
+  } else if constexpr (FMT != 2 && PL == 2) {
+    d2_t v = {convert<FMT>(r[0]), convert<FMT>(r[1])};
+    __builtin_nontemporal_store(v, reinterpret_cast<d2_t*>(p));
+  } else {
+#pragma unroll
+    for (int i = 0; i < PL; ++i) __builtin_nontemporal_store(convert<FMT>(r[i]), p + i);
+  }
+}
+
+// Generates `cnt` outputs of the stream whose wave-uniform state (at the
+// first output) is s, into dst[0..cnt).  `cnt` and s must be wave-uniform.
+template <int FMT, int PL>
+__device__ __forceinline__ void generate_run(typename Out<FMT>::T* __restrict__ dst, uint64_t cnt, uint32_t s[8],
+                                             const CoefRecord* __restrict__ lane_rec) {
+  constexpr int kPer = 64 * PL;
+  const int lane = threadIdx.x & 63;
+  typename Out<FMT>::T* p = dst + PL * lane;
+  const uint64_t full = cnt / kPer;
+  const uint64_t rem = cnt - full * kPer;
+  uint32_t any = 0;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) any |= s[j];
+  if (any == 0u) {  // degenerate all-zero state
+    for (uint64_t i = lane; i < cnt; i += 64) __builtin_nontemporal_store(convert_zero<FMT>(), dst + i);
+    return;
+  }
+  Coef k[PL];
+#pragma unroll
+  for (int i = 0; i < PL; ++i) k[i] = load_coef(lane_rec + PL * lane + i);
+  for (uint64_t it = 0; it < full; ++it) {
+    int32_t e[8];
+    centre(s, e);
+    uint32_t r[PL];
+#pragma unroll
+    for (int i = 0; i < PL; ++i) r[i] = dot_residue(k[i], e);
+    store_lane<FMT, PL>(p, r);
+    p += kPer;
+    // next state: the last eight words, newest first
+#pragma unroll
+    for (int j = 0; j < 8; ++j) s[j] = __builtin_amdgcn_readlane(r[(kPer - 1 - j) % PL], (kPer - 1 - j) / PL) + 1u;
+  }
+  if (rem) {
+    int32_t e[8];
+    centre(s, e);
+#pragma unroll
+    for (int i = 0; i < PL; ++i) {
+      const uint32_t r = dot_residue(k[i], e);
+      if ((uint64_t)(PL * lane + i) < rem) __builtin_nontemporal_store(convert<FMT>(r), p + i);
+    }
+  }
+}
+
+template <int FMT, int PL>
+__global__ __launch_bounds__(1024) void k_fill(FillArgs a) {
+  // wave index, made explicitly uniform so the loop state stays scalar
+  const uint32_t wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const uint64_t wave = (uint64_t)blockIdx.x * (blockDim.x >> 6) + wib;
+  const uint64_t first = wave * a.tile;
+  if (first >= a.n) return;
+  const uint64_t cnt = min(a.tile, a.n - first);
+  uint32_t s[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) s[j] = a.start.state[j];
+  const uint64_t off = a.start.offset + first;
+  if (off) jump_uniform(s, off, a.jump);
+  generate_run<FMT, PL>(static_cast<typename Out<FMT>::T*>(a.out) + first, cnt, s, a.lane);
+}
+
+// Persistent batched kernel: waves take tiles from a global counter until the
+// list is exhausted (every wave exits once the counter passes ntiles).
+template <int FMT>
+__device__ __forceinline__ void batch_tile(const BatchSegment& sg, uint64_t first, uint64_t cnt, uint32_t s[8],
+                                           const CoefRecord* __restrict__ lane_rec) {
+  typedef typename Out<FMT>::T T;
+  T* dst = static_cast<T*>(sg.out) + first;
+  const uintptr_t addr = reinterpret_cast<uintptr_t>(dst);
+  if constexpr (FMT == 2) {
+    if ((addr & 15u) == 0) return generate_run<FMT, 4>(dst, cnt, s, lane_rec);
+    if ((addr & 7u) == 0) return generate_run<FMT, 2>(dst, cnt, s, lane_rec);
+    return generate_run<FMT, 1>(dst, cnt, s, lane_rec);
+  } else {
+    if ((addr & 15u) == 0) return generate_run<FMT, 2>(dst, cnt, s, lane_rec);
+    return generate_run<FMT, 1>(dst, cnt, s, lane_rec);
+  }
+}
+
+__global__ __launch_bounds__(1024) void k_batch(BatchArgs a) {
+  const int lane = threadIdx.x & 63;
+  for (;;) {
+    uint32_t t = 0;
+    if (lane == 0) t = atomicAdd(a.cursor, 1u);
+    const uint64_t tile = (uint64_t)__builtin_amdgcn_readfirstlane(t);
+    if (tile >= a.hdr->ntiles) return;
+    // segment holding this tile: last i with tile_begin[i] <= tile (empty
+    // segments share their begin with the next one and are skipped)
+    uint32_t lo = 0, hi = a.hdr->nsegs;
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (a.tile_begin[mid] <= tile) lo = mid; else hi = mid;
+    }
+    const BatchSegment sg = a.segs[lo];
+    const uint64_t local = tile - a.tile_begin[lo];
+    const uint64_t first = local * a.tile;
+    const uint64_t cnt = min(a.tile, sg.count - first);
+    uint32_t s[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) s[j] = __builtin_amdgcn_readfirstlane(sg.state[j]);
+    const uint64_t off = sg.offset + first;
+    if (off) jump_uniform(s, off, a.jump);
+    switch (__builtin_amdgcn_readfirstlane(sg.format)) {
+      case 0: batch_tile<0>(sg, first, cnt, s, a.lane); break;
+      case 1: batch_tile<1>(sg, first, cnt, s, a.lane); break;
+      default: batch_tile<2>(sg, first, cnt, s, a.lane); break;
+    }
+  }
+}
+
+__global__ void k_zero_u32(uint32_t* p, uint32_t count) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) p[i] = 0u;
+}
+
+template <int FMT, int PL>
+hipError_t fill_one(const FillArgs& a, int block_threads, hipStream_t stream) {
+  const uint64_t waves = (a.n + a.tile - 1) / a.tile;
+  const uint64_t wpb = (uint64_t)block_threads / 64;
+  const uint64_t blocks = (waves + wpb - 1) / wpb;
+  if (blocks == 0) return hipSuccess;
+  if (blocks > 0x7fffffffull) return hipErrorInvalidConfiguration;
+  hipLaunchKernelGGL((k_fill<FMT, PL>), dim3((uint32_t)blocks), dim3(block_threads), 0, stream, a);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_fill(const FillArgs& a, int format, int per_lane, int block_threads, hipStream_t stream) {
+  switch (format * 8 + per_lane) {
+    case 0 * 8 + 1: return fill_one<0, 1>(a, block_threads, stream);
+    case 0 * 8 + 2: return fill_one<0, 2>(a, block_threads, stream);
+    case 1 * 8 + 1: return fill_one<1, 1>(a, block_threads, stream);
+    case 1 * 8 + 2: return fill_one<1, 2>(a, block_threads, stream);
+    case 2 * 8 + 1: return fill_one<2, 1>(a, block_threads, stream);
+    case 2 * 8 + 2: return fill_one<2, 2>(a, block_threads, stream);
+    case 2 * 8 + 4: return fill_one<2, 4>(a, block_threads, stream);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_batch(const BatchArgs& a, int grid, int block_threads, hipStream_t stream) {
+  hipLaunchKernelGGL(k_batch, dim3(grid), dim3(block_threads), 0, stream, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_zero_u32(uint32_t* p, uint32_t count, hipStream_t stream) {
+  if (count == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_zero_u32, dim3((count + 255) / 256), dim3(256), 0, stream, p, count);
+  return hipGetLastError();
+}
+
+}  // namespace mrg8

@@ -1,0 +1,73 @@
+// Internal interface between the host library and the gfx950 kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "mrg8_math.hpp"
+
+namespace mrg8 {
+
+constexpr int kLaneRows = 256;   // block rows kept on the device: up to 4 outputs per lane
+constexpr int kJumpBits = 64;    // device jumps take 64-bit offsets
+
+// Per-device constant tables.
+struct DeviceTables {
+  CoefRecord* lane = nullptr;  // kLaneRows records: row 0 of A^(k+1), support 0xff
+  CoefRecord* jump = nullptr;  // kJumpBits * 8 records: row r of A^(2^b)
+};
+
+// Wave-uniform starting point of a launch.
+struct StreamStart {
+  uint32_t state[8];  // newest word first
+  uint64_t offset;    // jump applied before output 0
+};
+
+// One fill launch over a single stream: output i of the launch is stream
+// position start.offset + i.  Each wave produces `tile` consecutive outputs.
+struct FillArgs {
+  void* out;
+  uint64_t n;
+  uint64_t tile;            // multiple of 64 * outputs-per-lane
+  StreamStart start;
+  const CoefRecord* lane;
+  const CoefRecord* jump;
+};
+
+// Work item of the batched kernel: `count` outputs of segment `desc`
+// starting `first` outputs into the segment.
+struct BatchSegment {
+  uint32_t state[8];
+  uint64_t offset;   // position of output 0 of the segment relative to state
+  uint64_t count;
+  void* out;
+  int32_t format;
+  uint32_t pad;
+};
+static_assert(sizeof(BatchSegment) == 64, "segment layout");
+
+// Device-resident header of a batch: counts change per run without
+// re-capturing the launch.
+struct BatchHeader {
+  uint32_t nsegs;
+  uint32_t pad;
+  uint64_t ntiles;
+};
+
+struct BatchArgs {
+  const BatchHeader* hdr;
+  const BatchSegment* segs;
+  const uint64_t* tile_begin;  // prefix sum of tiles per segment, nsegs + 1 entries
+  uint64_t tile;               // outputs per tile
+  uint32_t* cursor;            // work counter, zeroed at the start of the graph
+  const CoefRecord* lane;
+  const CoefRecord* jump;
+};
+
+// Launchers (mrg8_device.hip).  All are asynchronous on `stream`.
+hipError_t launch_fill(const FillArgs& a, int format, int per_lane, int block_threads, hipStream_t stream);
+hipError_t launch_batch(const BatchArgs& a, int grid, int block_threads, hipStream_t stream);
+hipError_t launch_zero_u32(uint32_t* p, uint32_t count, hipStream_t stream);
+
+}  // namespace mrg8

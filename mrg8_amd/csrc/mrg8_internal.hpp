@@ -1,0 +1,39 @@
+// Shared host-side pieces of libmrg8_amd: per-device context, error helpers,
+// single-stream fill.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+
+#include "mrg8_device.hpp"
+
+namespace mrg8 {
+
+struct DeviceContext {
+  int device = 0;
+  int num_cu = 0;
+  DeviceTables tables;
+};
+
+// Records `fmt` as this thread's last error message and returns `code`.
+int fail(int code, const char* fmt, ...);
+int hip_fail(hipError_t e, const char* what);
+void clear_error();
+
+const JumpTable& host_jump_table();
+int device_context(int device, DeviceContext** out);
+bool state_valid(const uint32_t st[8]);
+int format_size(int format);
+int pick_per_lane(int format, const void* p);
+
+// Fills dptr[0..n) with stream positions offset..offset+n-1 of the stream
+// whose state is `state`; stream-ordered.  block_threads/tile 0 = default.
+int fill_device(DeviceContext& ctx, const uint32_t state[8], uint64_t offset, void* dptr, uint64_t n, int format,
+                hipStream_t stream, int block_threads, uint64_t tile);
+
+// Host wall clock in seconds (steady).
+double now_seconds();
+
+}  // namespace mrg8

@@ -1,0 +1,196 @@
+"""GPU parity: the gfx950 kernels against the oracle (plain-C restatement of
+original/mrg8.cpp), bit for bit, through the C ABI."""
+import numpy as np
+import pytest
+
+from conftest import wsum
+
+pytestmark = pytest.mark.gpu
+
+SEED = 13579  # the reference's test seed (cuda/tests/test_mrg8_outer.cu:35)
+NP_DTYPE = {0: np.float64, 1: np.float64, 2: np.uint32}
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+
+
+def run_device(gpu, g, n, fmt, pad_elems=0, **shape):
+    from mrg8_amd.hip import DeviceBuffer
+
+    dt = NP_DTYPE[fmt]
+    esz = np.dtype(dt).itemsize
+    buf = DeviceBuffer((n + pad_elems + 1) * esz)
+    g.generate(buf.ptr + pad_elems * esz, n, fmt, **shape)
+    g.synchronize()
+    out = buf.copy_to_host(dt, n, pad_elems * esz)
+    buf.free()
+    return out
+
+
+@pytest.mark.parametrize("fmt", [0, 1, 2])
+def test_c1_full_compare(gpu, oracle_mod, fmt):
+    """2^24 outputs from seed 13579 compared in full (BASELINE config 1 shape)."""
+    n = 1 << 24
+    say("c1", fmt, n)
+    g = gpu.mrg8_cuda(SEED)
+    got = run_device(gpu, g, n, fmt)
+    want, st = oracle_mod.generate(oracle_mod.seed_state(SEED), n, fmt)
+    bad = np.flatnonzero(got != want) if fmt == 2 else np.flatnonzero(got.view(np.uint64) != want.view(np.uint64))
+    assert bad.size == 0, f"first mismatch at {bad[:5]}: got {got[bad[:5]]} want {want[bad[:5]]}"
+    assert np.array_equal(g.get_state(), st)
+
+
+@pytest.mark.parametrize("n", [1, 7, 8, 9, 63, 64, 65, 127, 128, 129, 255, 1000, 4095, 65537, 300001])
+@pytest.mark.parametrize("fmt", [0, 1, 2])
+def test_ragged_sizes(gpu, oracle_mod, n, fmt):
+    say("ragged", n, fmt)
+    g = gpu.mrg8_cuda(SEED + n)
+    got = run_device(gpu, g, n, fmt)
+    want, st = oracle_mod.generate(oracle_mod.seed_state(SEED + n), n, fmt)
+    assert np.array_equal(got.view(np.uint8), want.view(np.uint8))
+    assert np.array_equal(g.get_state(), st)
+
+
+@pytest.mark.parametrize("pad", [1, 2, 3])
+@pytest.mark.parametrize("fmt", [0, 2])
+def test_unaligned_outputs(gpu, oracle_mod, pad, fmt):
+    """Outputs that are only element-aligned take the narrow-store path."""
+    n = 100003
+    say("unaligned", pad, fmt)
+    g = gpu.mrg8_cuda(SEED)
+    got = run_device(gpu, g, n, fmt, pad_elems=pad)
+    want, _ = oracle_mod.generate(oracle_mod.seed_state(SEED), n, fmt)
+    assert np.array_equal(got.view(np.uint8), want.view(np.uint8))
+
+
+@pytest.mark.parametrize("block,tile", [(64, 128), (256, 1024), (512, 4096), (1024, 65536), (256, 1 << 20)])
+def test_launch_shapes_identical(gpu, oracle_mod, block, tile):
+    n = 1_000_003
+    say("shape", block, tile)
+    g = gpu.mrg8_cuda(SEED)
+    got = run_device(gpu, g, n, 0, block_threads=block, tile=tile)
+    want, _ = oracle_mod.generate(oracle_mod.seed_state(SEED), n, 0)
+    assert np.array_equal(got.view(np.uint64), want.view(np.uint64))
+
+
+def test_consecutive_calls_continue(gpu, oracle_mod):
+    say("consecutive")
+    g = gpu.mrg8_cuda(SEED)
+    parts = [run_device(gpu, g, n, 2) for n in (5, 1000, 77, 4096, 123457)]
+    want, st = oracle_mod.generate(oracle_mod.seed_state(SEED), sum(p.size for p in parts), 2)
+    assert np.array_equal(np.concatenate(parts), want)
+    assert np.array_equal(g.get_state(), st)
+
+
+def test_set_state_and_jump(gpu, oracle_mod):
+    rng = np.random.default_rng(7)
+    for trial in range(20):
+        st = rng.integers(1, 2**31 - 1, size=8, dtype=np.uint64).astype(np.uint32)
+        step = int(rng.integers(0, 2**63, dtype=np.uint64))
+        say("jump", trial, step)
+        g = gpu.mrg8_cuda(1)
+        g.set_state(st)
+        g.jump_ahead(step)
+        got = run_device(gpu, g, 5000, 2)
+        want, _ = oracle_mod.generate(oracle_mod.jump(st, step), 5000, 2)
+        assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("words", [
+    [0, 0, 0, 0, 0, 0, 0, 5],
+    [0, 2147483647, 0, 0, 0, 0, 0, 0],
+    [2147483647] * 8,
+    [0] * 8,
+    [1, 0, 1, 0, 1, 0, 1, 0],
+])
+@pytest.mark.parametrize("fmt", [0, 1, 2])
+def test_noncanonical_states(gpu, oracle_mod, words, fmt):
+    """Zero and p words in the state (both allowed by set_state) keep the
+    reference's representation through the kernels' jump and generation."""
+    say("noncanonical", words, fmt)
+    w = np.array(words, dtype=np.uint32)
+    for offset in (0, 1, 3, 9, 100):
+        g = gpu.mrg8_cuda(1)
+        g.set_state(w)
+        g.jump_ahead(offset)
+        n = 3000
+        # tiny tiles: every wave after the first starts with a device jump
+        got = run_device(gpu, g, n, fmt, block_threads=64, tile=64)
+        want, st = oracle_mod.generate(oracle_mod.jump(w, offset), n, fmt)
+        assert np.array_equal(got.view(np.uint8), want.view(np.uint8)), offset
+        assert np.array_equal(g.get_state(), st)
+
+
+def test_device_jump_small_offsets(gpu, oracle_mod):
+    """Waves that start at small offsets (1..8 positions) exercise the shift rows of A, A^2, A^4."""
+    for n in range(129, 129 + 16):
+        say("small-offset", n)
+        g = gpu.mrg8_cuda(SEED)
+        got = run_device(gpu, g, n, 2, block_threads=64, tile=128)
+        want, _ = oracle_mod.generate(oracle_mod.seed_state(SEED), n, 2)
+        assert np.array_equal(got, want)
+
+
+def test_drop_in_outer_inner(gpu, oracle_mod):
+    from mrg8_amd.hip import DeviceBuffer
+
+    n = 1 << 20
+    say("drop-in")
+    buf = DeviceBuffer(n * 8)
+    for call in ("outer", "inner"):
+        g = gpu.mrg8_cuda(SEED)
+        if call == "outer":
+            g.mrg8_outer_32(buf, n, 64, 65536)
+        else:
+            g.mrg8_inner(buf, n, 65536)
+        got = buf.copy_to_host(np.float64, n)
+        want, st = oracle_mod.generate(oracle_mod.seed_state(SEED), n, 1)
+        assert np.array_equal(got.view(np.uint64), want.view(np.uint64)), call
+        g.state_cpy_DtH()
+        assert np.array_equal(g.get_state(), st)
+    buf.free()
+
+
+def test_host_forms(gpu, oracle_mod):
+    say("host-forms")
+    g = gpu.mrg8_cuda(SEED)
+    a = g.randint(1000)
+    b = g.rand(2000)
+    c = g.mrg8dnz2(3000)
+    d = g.randint()
+    e = g.rand()
+    want, _ = oracle_mod.generate(oracle_mod.seed_state(SEED), 6002, 2)
+    rn = 1.0 / 2147483647.0
+    assert np.array_equal(a, want[:1000])
+    assert np.array_equal(b, want[1000:3000].astype(np.float64) * rn)
+    assert np.array_equal(c, want[3000:6000].astype(np.float64) * rn)
+    assert d == int(want[6000])
+    assert e == float(want[6001]) * rn
+
+
+def test_c2_sampled_and_wsum(gpu, oracle_mod):
+    """2^28 outputs in u32 and f64: 1e4 sampled indices against jump+1 output,
+    and a position-weighted checksum of the whole sequence."""
+    from mrg8_amd.hip import DeviceBuffer
+
+    n = 1 << 28
+    rng = np.random.default_rng(12345)
+    idx = np.sort(rng.integers(0, n, size=10_000, dtype=np.uint64))
+    s0 = oracle_mod.seed_state(SEED)
+    for fmt in (2, 0):
+        say("c2", fmt)
+        dt = NP_DTYPE[fmt]
+        buf = DeviceBuffer(n * np.dtype(dt).itemsize)
+        g = gpu.mrg8_cuda(SEED)
+        g.generate(buf, n, fmt)
+        g.synchronize()
+        got = buf.copy_to_host(dt, n)
+        buf.free()
+        for i in idx:
+            v, _ = oracle_mod.generate(oracle_mod.jump(s0, int(i)), 1, fmt)
+            assert got[int(i)].tobytes() == v[0].tobytes(), int(i)
+        h, st = oracle_mod.hash_stream(s0, n, fmt)
+        assert oracle_mod.fnv1a(got) == h
+        assert np.array_equal(g.get_state(), st)
+        del got

@@ -1,0 +1,138 @@
+"""CPU-side checks of libmrg8_amd.so: it loads, exports every entry point the
+public headers declare, and its host logic (seeding, jumps, jump table) agrees
+with the oracle and the golden vectors.  No device work is done here."""
+import ctypes
+import hashlib
+import json
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLD = json.load(open(os.path.join(ROOT, "tests", "golden", "ref_vectors.json")))
+
+
+@pytest.fixture(scope="module")
+def lib():
+    from mrg8_amd import capi
+
+    return capi
+
+
+def header_functions():
+    names = set()
+    inc = os.path.join(ROOT, "include")
+    for f in os.listdir(inc):
+        if not f.endswith(".h"):
+            continue
+        text = open(os.path.join(inc, f)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"^\s*(?:const\s+)?\w+\s*\*?\s*(mrg8_\w+)\s*\(", text, flags=re.M):
+            names.add(m.group(1))
+    return names
+
+
+def test_exports_every_declared_function(lib):
+    names = header_functions()
+    assert len(names) > 30
+    missing = [n for n in sorted(names) if not hasattr(lib.lib(), n)]
+    assert not missing, missing
+    # and the Python bindings cover them all
+    assert names <= set(lib.SIGNATURES), sorted(names - set(lib.SIGNATURES))
+
+
+def test_library_is_gfx950_only():
+    """The shared object's offload bundle carries gfx950 code objects only."""
+    so = open(os.path.join(ROOT, "mrg8_amd", "libmrg8_amd.so"), "rb").read()
+    targets = set(re.findall(rb"hipv4-amdgcn-amd-amdhsa--(gfx[0-9a-z]+)", so))
+    assert targets == {b"gfx950"}, targets
+
+
+def test_version_and_strerror(lib):
+    L = lib.lib()
+    assert L.mrg8_abi_version() == 1
+    assert L.mrg8_strerror(0) == b"ok"
+    assert L.mrg8_strerror(-1) == b"invalid argument"
+
+
+def test_seed_state_matches_golden(oracle_mod):
+    import mrg8_amd
+
+    for s, want in GOLD["seed_states"].items():
+        assert mrg8_amd.seed_state(int(s)).tolist() == want
+
+
+def test_advance_state_matches_golden_and_oracle(oracle_mod):
+    import mrg8_amd
+
+    s0 = mrg8_amd.seed_state(GOLD["seed"])
+    for j in GOLD["jumps"]:
+        assert mrg8_amd.advance_state(s0, int(j["step"])).tolist() == j["state"], j["step"]
+    for j in GOLD["jump_bits"]:
+        assert mrg8_amd.advance_state(s0, bits=j["bits"]).tolist() == j["state"], j["bits"]
+    rng = np.random.default_rng(9)
+    for t in range(300):
+        st = rng.integers(0, 2**31, size=8).astype(np.uint32)
+        if t % 3 == 0:
+            st[rng.integers(0, 8, size=int(rng.integers(1, 8)))] = rng.choice([0, 2**31 - 1])
+        step = int(rng.integers(0, 2**64, dtype=np.uint64)) if t % 2 else int(rng.integers(0, 32))
+        assert np.array_equal(mrg8_amd.advance_state(st, step), oracle_mod.jump(st, step)), (st, step)
+
+
+def test_advance_equals_generation(oracle_mod):
+    import mrg8_amd
+
+    s0 = mrg8_amd.seed_state(GOLD["seed"])
+    for n, want in GOLD["states_after"].items():
+        assert mrg8_amd.advance_state(s0, int(n)).tolist() == want, n
+
+
+def test_noncanonical_jumps_match_golden():
+    import mrg8_amd
+
+    for r in GOLD["noncanonical"]:
+        st = mrg8_amd.advance_state(r["init"], int(r["step"]))
+        # the next 300 outputs end in r["state"]: advancing 300 more must agree
+        assert mrg8_amd.advance_state(st, 300).tolist() == r["state"]
+
+
+def test_jump_matrix_and_file(tmp_path, oracle_mod):
+    import mrg8_amd
+
+    assert np.array_equal(mrg8_amd.jump_matrix(), oracle_mod.jump_table())
+    p = tmp_path / "jump_matrix.txt"
+    mrg8_amd.write_jump_matrix(str(p))
+    assert hashlib.md5(p.read_bytes()).hexdigest() == GOLD["jump_matrix_md5"]
+
+
+def test_argument_errors(lib):
+    import mrg8_amd
+
+    with pytest.raises(lib.Mrg8Error) as e:
+        mrg8_amd.advance_state([2**31] + [1] * 7, 5)  # words above p are rejected
+    assert e.value.code == lib.ERR_ARG
+    L = lib.lib()
+    assert L.mrg8_seed_state(1, None) == lib.ERR_ARG
+    assert L.mrg8_advance_state(None, 1, None) == lib.ERR_ARG
+    assert b"null" in L.mrg8_last_error()
+    assert L.mrg8_write_jump_matrix(b"/nonexistent-dir/x.txt") == lib.ERR_IO
+    assert L.mrg8_jump_matrix(None, 0) == lib.ERR_ARG
+    buf = (ctypes.c_uint32 * 10)()
+    assert L.mrg8_jump_matrix(buf, 10) == lib.ERR_ARG
+
+
+def test_no_device_is_an_error_not_a_fallback(lib):
+    """Without a GPU, device entry points fail with MRG8_ERR_NODEVICE."""
+    from mrg8_amd import hip
+
+    if hip.device_count() > 0:
+        pytest.skip("a device is present")
+    import mrg8_amd
+
+    with pytest.raises(lib.Mrg8Error) as e:
+        mrg8_amd.mrg8_cuda(1)
+    assert e.value.code == lib.ERR_NODEVICE
+    h = ctypes.c_void_p()
+    assert lib.lib().mrg8_batch_create(0, 4, 16, ctypes.byref(h)) in (lib.ERR_NODEVICE, lib.ERR_ARG)
