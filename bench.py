@@ -1,0 +1,300 @@
+"""MRG8 benchmark on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--quick]
+
+A step is one generation of 2^28 float64 outputs (x * (1/(2^31-1)), the
+reference's `rand` conversion) of a single stream seeded with 13579, split
+across all waves by jump-ahead -- the single-stream workload of the
+benchmark configurations.  Printed value: G samples/s over the timed steps.
+
+The same JSON line carries, measured in the same run:
+  roofline      algorithmic bytes per launch / mean kernel time (HIP events on
+                the launch stream) against 8 TB/s; HBM traffic from the
+                committed rocprofv3 counter summary when present
+  u32           the same workload in raw uint32 words
+  small         2^24 float64 outputs (the reference's full-compare size)
+  batched       4096 streams, log-uniform lengths in [2^10, 2^22), float64, one
+                persistent graph-captured launch with the autotuned shape
+  jumps         65536 streams jumped by uniform 63-bit steps, 256 uint32 each
+  served        1e5 Poisson requests on 16384 resident streams, sizes
+                log-uniform in [2^8, 2^26) float64, 64 GB admission cap:
+                throughput and p50/p99 latency (arrival to completion)
+  cpu_baseline  the reference's serial generator (original/mrg8.cpp, compiled
+                from its sources) on one host core, bounded sample
+  cpu_baseline_vec  the knl vector/OpenMP restatement on all host cores
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "G samples/s & % HBM write roofline (1 stream, batched); p99 latency served"
+PEAK_GBS = 8000.0
+N_SINGLE = 1 << 28
+SEED = 13579
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def time_launches(fn, stream, reps):
+    """Mean ms per launch from HIP events recorded around each launch on `stream`."""
+    from mrg8_amd import hip
+
+    evs = [(hip.Event(), hip.Event()) for _ in range(reps)]
+    for a, b in evs:
+        a.record(stream)
+        fn()
+        b.record(stream)
+    evs[-1][1].synchronize()
+    ms = [a.elapsed_ms(b) for a, b in evs]
+    for a, b in evs:
+        a.destroy()
+        b.destroy()
+    return float(np.mean(ms)), float(np.min(ms))
+
+
+def traffic_from_profiles(name):
+    """HBM bytes per launch recorded by the committed counter summary, if any."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            try:
+                return float(line.split(":", 1)[1].split()[0])
+            except ValueError:
+                return None
+    return None
+
+
+def bench_single(args):
+    import mrg8_amd
+    from mrg8_amd import hip
+
+    s = hip.Stream()
+    buf = hip.DeviceBuffer(N_SINGLE * 8)
+    g = mrg8_amd.mrg8_cuda(SEED)
+    out = {}
+    for fmt, name, esz in ((mrg8_amd.F64, "f64", 8), (mrg8_amd.U32, "u32", 4)):
+        step = lambda: g.generate(buf, N_SINGLE, fmt, stream=s)  # noqa: E731
+        for _ in range(args.warmup):
+            step()
+        s.synchronize()
+        hip.synchronize()
+        t0 = time.perf_counter()
+        e0, e1 = hip.Event(), hip.Event()
+        e0.record(s)
+        for _ in range(args.steps):
+            step()
+        e1.record(s)
+        e1.synchronize()
+        hip.synchronize()
+        wall = time.perf_counter() - t0
+        ev_s = e0.elapsed_ms(e1) / 1e3
+        kern_ms, kern_min = time_launches(step, s, max(3, min(args.steps, 10)))
+        out[name] = {"wall_s": wall, "event_s": ev_s, "kernel_ms": kern_ms, "kernel_ms_min": kern_min,
+                     "bytes": N_SINGLE * esz}
+        log(f"single {name}: {N_SINGLE * args.steps / wall / 1e9:.1f} G samples/s wall, kernel {kern_ms:.3f} ms")
+    # the reference's full-compare size
+    small = 1 << 24
+    step = lambda: g.generate(buf, small, mrg8_amd.F64, stream=s)  # noqa: E731
+    for _ in range(3):
+        step()
+    km, kmin = time_launches(step, s, 20)
+    out["small"] = {"kernel_ms": km, "gsps": small / (km * 1e-3) / 1e9,
+                    "gbs": small * 8 / (km * 1e-3) / 1e9}
+    buf.free()
+    s.destroy()
+    return out
+
+
+def bench_batched(args):
+    import mrg8_amd
+    from mrg8_amd import hip, workloads
+
+    states, lengths = workloads.batched()
+    total = int(lengths.sum())
+    buf = hip.DeviceBuffer(total * 8)
+    starts = np.concatenate([[0], np.cumsum(lengths)[:-1]]).astype(np.uint64)
+    b = mrg8_amd.Batch(max_streams=len(lengths))
+    b.set([(states[i], 0, int(lengths[i]), buf.ptr + 8 * int(starts[i]), mrg8_amd.F64) for i in range(len(lengths))])
+    s = hip.Stream()
+    cfg = b.autotune(stream=s, cache_path=os.path.join(tempfile.gettempdir(), "mrg8_batch_tune.txt"))
+    for _ in range(2):
+        b.run(s)
+    km, kmin = time_launches(lambda: b.run(s), s, max(5, args.steps))
+    buf.free()
+    gbs = total * 8 / (km * 1e-3) / 1e9
+    log(f"batched: {total / (km * 1e-3) / 1e9:.1f} G samples/s, {gbs:.0f} GB/s, cfg {cfg}")
+    return {"streams": len(lengths), "outputs": total, "ms_per_launch": km, "gsps": total / (km * 1e-3) / 1e9,
+            "gbs": gbs, "frac_of_peak": gbs / PEAK_GBS, "config": cfg, "launch": "persistent, hipGraph replay"}
+
+
+def bench_jumps(args):
+    import mrg8_amd
+    from mrg8_amd import hip, workloads
+
+    steps = workloads.jump_steps()
+    n = len(steps)
+    buf = hip.DeviceBuffer(n * 256 * 4)
+    s0 = mrg8_amd.seed_state(SEED)
+    b = mrg8_amd.Batch(max_streams=n, max_tiles=n)
+    b.set_config(256, 8, 256)
+    b.set([(s0, int(steps[i]), 256, buf.ptr + 1024 * i, mrg8_amd.U32) for i in range(n)])
+    s = hip.Stream()
+    for _ in range(2):
+        b.run(s)
+    km, _ = time_launches(lambda: b.run(s), s, max(5, args.steps))
+    buf.free()
+    pc = float(np.mean([bin(int(x)).count("1") for x in steps]))
+    log(f"jumps: {km:.3f} ms per launch, {n / (km * 1e-3) / 1e6:.1f} M streams/s")
+    return {"streams": n, "outputs_per_stream": 256, "ms_per_launch": km, "streams_per_s": n / (km * 1e-3),
+            "gsps": n * 256 / (km * 1e-3) / 1e9, "mean_popcount": pc,
+            "jump_macs_per_s": n * pc * 64 / (km * 1e-3)}
+
+
+def bench_served(args, capacity_gsps):
+    import mrg8_amd
+    from mrg8_amd import workloads
+
+    n_req = args.served_requests
+    mean = workloads.mean_served_size()
+    rate = args.served_load * capacity_gsps * 1e9 / mean  # requests per second at the offered load
+    states, arrivals, ids, sizes = workloads.served(n_requests=n_req, rate_per_s=rate)
+    srv = mrg8_amd.Server(num_streams=len(states), max_output_bytes=64 << 30, max_inflight=4096,
+                          step_outputs=args.step_outputs)
+    for i in range(len(states)):
+        srv.set_stream(i, states[i], 0)
+    # warm-up trace (separate streams' positions advance; states stay resident)
+    srv.replay(arrivals[:200] / 10, ids[:200], sizes[:200], fmt=mrg8_amd.F64)
+    stats = srv.replay(arrivals, ids, sizes, fmt=mrg8_amd.F64)
+    srv.close()
+    out = {"requests": n_req, "streams": len(states), "offered_load": args.served_load,
+           "arrival_rate_per_s": rate, "outputs": int(stats["outputs"]), "wall_s": stats["wall_s"],
+           "gsps": stats["outputs"] / stats["wall_s"] / 1e9, "p50_ms": stats["p50_s"] * 1e3,
+           "p90_ms": stats["p90_s"] * 1e3, "p99_ms": stats["p99_s"] * 1e3, "max_ms": stats["max_s"] * 1e3,
+           "admission_cap_bytes": 64 << 30, "step_outputs": args.step_outputs}
+    log(f"served: {out['gsps']:.1f} G samples/s, p50 {out['p50_ms']:.3f} ms, p99 {out['p99_ms']:.3f} ms")
+    return out
+
+
+def cpu_baselines(args):
+    """The oracle/_ref reference build (1 core) and the vector port (all cores)."""
+    import oracle
+
+    res = {}
+    n = 1 << args.cpu_log2
+    ref = os.path.join(ROOT, "oracle", "_ref", "ref_original")
+    if os.path.exists(ref) and os.path.exists(os.path.join(ROOT, "oracle", "_ref", "jump_matrix.txt")):
+        r = subprocess.run([ref], cwd=os.path.dirname(ref), input=f"T {SEED} {n} 10\nQ\n", capture_output=True,
+                           text=True, timeout=600)
+        ms = [float(x) for x in r.stdout.split()]
+        mean = float(np.mean(ms[1:]))  # the reference's method: drop the first of 10, mean of 9
+        res["cpu_baseline"] = {"value": n / (mean * 1e-3) / 1e9, "unit": "G samples/s", "cores": 1,
+                               "kind": "reference",
+                               "sample": f"original/mrg8.cpp rand(double*, 2^{args.cpu_log2}) from seed {SEED}, "
+                                         f"10 runs, first dropped, mean of 9 = {mean:.1f} ms"}
+    else:
+        st = oracle.seed_state(SEED)
+        buf = np.empty(n, dtype=np.float64)
+        ts = []
+        for _ in range(4):
+            t0 = time.perf_counter()
+            oracle.lib().mrg8o_rand(oracle._ptr(st.copy()), buf.ctypes.data, n)
+            ts.append(time.perf_counter() - t0)
+        mean = float(np.mean(ts[1:]))
+        res["cpu_baseline"] = {"value": n / mean / 1e9, "unit": "G samples/s", "cores": 1, "kind": "port",
+                               "sample": f"oracle serial rand, 2^{args.cpu_log2} outputs, mean of 3"}
+    # vector + OpenMP restatement of the knl path on every host core
+    nv = 1 << 28
+    buf = np.empty(nv, dtype=np.float64)
+    st = oracle.seed_state(SEED)
+    ts, used = [], 0
+    for _ in range(10):
+        t0 = time.perf_counter()
+        _, _, used = oracle.vec_outer_tp(st, nv, 0, out=buf)
+        ts.append(time.perf_counter() - t0)
+    mean = float(np.mean(ts[1:]))
+    res["cpu_baseline_vec"] = {"value": nv / mean / 1e9, "unit": "G samples/s", "cores": used, "kind": "port",
+                               "avx512": oracle.vec_has_avx512(),
+                               "sample": "knl/mrg8_vec.h outer-product restatement, 2^28 (x-1)/p doubles, "
+                                         "10 runs, first dropped, mean of 9"}
+    try:
+        res["host_cpu"] = subprocess.run(["sh", "-c", "grep -m1 'model name' /proc/cpuinfo | cut -d: -f2"],
+                                         capture_output=True, text=True).stdout.strip()
+    except Exception:
+        pass
+    res["host_nproc"] = os.cpu_count()
+    return res
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--quick", action="store_true", help="single-stream figures only")
+    ap.add_argument("--served-requests", type=int, default=100_000)
+    ap.add_argument("--served-load", type=float, default=0.7)
+    ap.add_argument("--step-outputs", type=int, default=1 << 26)
+    ap.add_argument("--cpu-log2", type=int, default=26)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the generator needs no "
+              f"cross-GPU exchange; see DESIGN.md for how it shards)", file=sys.stderr)
+        sys.exit(2)
+
+    import mrg8_amd
+    from mrg8_amd import hip
+
+    if hip.device_count() < 1:
+        print("no HIP device", file=sys.stderr)
+        sys.exit(1)
+    mrg8_amd.capi.call("mrg8_init_device", 0)
+
+    single = bench_single(args)
+    f64 = single["f64"]
+    value = N_SINGLE * args.steps / f64["wall_s"] / 1e9
+    achieved = f64["bytes"] / (f64["kernel_ms"] * 1e-3) / 1e9
+    traffic = traffic_from_profiles("fill_f64_2e28_counters.txt")
+    line = {
+        "metric": METRIC, "value": round(value, 2), "unit": "G samples/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(f64["wall_s"] / args.steps * 1e3, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "int64", "data": "synthetic",
+        "config": {"workload": "single stream, seed 13579, 2^28 float64 outputs double(x)*(1/(2^31-1)), split "
+                               "across all waves by jump-ahead (exact 64-bit integer arithmetic mod 2^31-1)",
+                   "outputs": N_SINGLE, "output_dtype": "float64", "parallelism": "1 GPU"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / PEAK_GBS, 4),
+                     "traffic": traffic,
+                     "kernel": "k_fill<F64,2>", "kernel_ms": round(f64["kernel_ms"], 4),
+                     "algorithmic_bytes_per_launch": f64["bytes"]},
+        "u32": {"gsps": round(N_SINGLE * args.steps / single["u32"]["wall_s"] / 1e9, 2),
+                "kernel_ms": round(single["u32"]["kernel_ms"], 4),
+                "gbs": round(single["u32"]["bytes"] / (single["u32"]["kernel_ms"] * 1e-3) / 1e9, 1)},
+        "small_2e24_f64": {k: round(v, 4) for k, v in single["small"].items()},
+    }
+    if not args.quick:
+        line["batched"] = bench_batched(args)
+        line["jumps"] = bench_jumps(args)
+        line["served"] = bench_served(args, line["batched"]["gsps"])
+    if not args.no_cpu:
+        line.update(cpu_baselines(args))
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -56,6 +56,7 @@ extern "C" {
 
 typedef struct mrg8_gen* mrg8_gen_t;
 typedef struct mrg8_batch* mrg8_batch_t;
+typedef struct mrg8_server* mrg8_server_t;
 
 /* ---- library ----------------------------------------------------------- */
 
@@ -187,6 +188,73 @@ int mrg8_batch_run(mrg8_batch_t b, void* stream);
  * rewritten each trial) and keeps the fastest.  `cache_path` (may be NULL)
  * is a text file of results keyed by device and workload class. */
 int mrg8_batch_autotune(mrg8_batch_t b, void* stream, const char* cache_path, mrg8_batch_config* best);
+
+/* ---- serving: resident streams, continuous batching -------------------- */
+
+typedef struct mrg8_server_config {
+  uint32_t num_streams;        /* resident per-stream states in device memory */
+  uint32_t max_inflight;       /* requests admitted at once */
+  uint64_t max_output_bytes;   /* admission cap on output-buffer bytes in flight */
+  uint64_t step_outputs;       /* target outputs per launch (0 = automatic) */
+  uint64_t tile_outputs;       /* outputs per work item (0 = automatic) */
+  int32_t block_threads;       /* 0 = automatic */
+  int32_t blocks_per_cu;       /* 0 = automatic */
+} mrg8_server_config;
+
+typedef struct mrg8_completion {
+  uint64_t request_id;
+  uint32_t stream_id;
+  int32_t status;         /* MRG8_OK or an error */
+  uint64_t start;         /* stream position of the first output */
+  uint64_t count;
+  void* out;              /* device buffer holding the outputs (owned by the server) */
+  double submit_time_s;   /* host clock (seconds) at submit */
+  double complete_time_s; /* host clock at which completion was observed */
+} mrg8_completion;
+
+/* Creates a server on `device` with `num_streams` resident states; stream i
+ * starts from the state of seed (base_seed + i) unless set otherwise. */
+int mrg8_server_create(int device, const mrg8_server_config* cfg, uint32_t base_seed, mrg8_server_t* out);
+int mrg8_server_destroy(mrg8_server_t s);
+int mrg8_server_set_stream(mrg8_server_t s, uint32_t stream_id, const uint32_t state[8], uint64_t position);
+int mrg8_server_get_stream(mrg8_server_t s, uint32_t stream_id, uint32_t state[8], uint64_t* position);
+/* Queues a request for the next `count` outputs of stream `stream_id`; the
+ * request is given the stream's next `count` positions at submit time, so
+ * its outputs do not depend on what else is in flight.  Admission to the GPU
+ * happens inside mrg8_server_progress as output memory and slots free up. */
+int mrg8_server_submit(mrg8_server_t s, uint32_t stream_id, uint64_t count, int format, uint64_t* request_id);
+/* Admits queued requests, launches work, and collects finished requests.
+ * Up to `max` completions are written to `done`; *ndone gets the number.
+ * `wait_ms` > 0 blocks up to that long for at least one completion. */
+int mrg8_server_progress(mrg8_server_t s, mrg8_completion* done, uint32_t max, uint32_t* ndone, int wait_ms);
+/* Returns a completed request's output buffer to the admission pool. */
+int mrg8_server_release(mrg8_server_t s, uint64_t request_id);
+/* Counters: queued, admitted (in flight), completed, output bytes in flight. */
+int mrg8_server_stats(mrg8_server_t s, uint64_t* queued, uint64_t* inflight, uint64_t* completed,
+                      uint64_t* bytes_inflight);
+
+/* Trace replay: a load generator for the served benchmark.  Requests are
+ * submitted when the wall clock passes their arrival time (seconds from the
+ * start of the replay, non-decreasing), each completion is released at once,
+ * and latency is measured from arrival to observed completion, so it
+ * includes queueing.  `latency_s` (n entries, may be NULL) receives the
+ * per-request latencies in trace order. */
+typedef struct mrg8_trace_entry {
+  double arrival_s;
+  uint32_t stream_id;
+  int32_t format;
+  uint64_t count;
+} mrg8_trace_entry;
+
+typedef struct mrg8_trace_stats {
+  uint64_t completed;
+  uint64_t outputs;
+  double wall_s;
+  double p50_s, p90_s, p99_s, max_s, mean_s;
+} mrg8_trace_stats;
+
+int mrg8_server_replay(mrg8_server_t s, const mrg8_trace_entry* trace, uint64_t n, double* latency_s,
+                       mrg8_trace_stats* stats);
 
 #ifdef __cplusplus
 }

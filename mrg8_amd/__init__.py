@@ -240,3 +240,77 @@ class Batch:
         capi.call("mrg8_batch_autotune", self._h, _stream(stream), cache_path.encode() if cache_path else None,
                   ctypes.byref(c))
         return {"block_threads": c.block_threads, "blocks_per_cu": c.blocks_per_cu, "tile_outputs": c.tile_outputs}
+
+
+class Server:
+    """Serving loop with device-resident stream states (see include/mrg8.h)."""
+
+    def __init__(self, num_streams: int, max_output_bytes: int, base_seed: int = 1, max_inflight: int = 4096,
+                 step_outputs: int = 0, tile_outputs: int = 0, block_threads: int = 0, blocks_per_cu: int = 0,
+                 device: int = 0):
+        cfg = capi.ServerConfig(num_streams, max_inflight, max_output_bytes, step_outputs, tile_outputs,
+                                block_threads, blocks_per_cu)
+        h = ctypes.c_void_p()
+        capi.call("mrg8_server_create", device, ctypes.byref(cfg), ctypes.c_uint32(base_seed), ctypes.byref(h))
+        self._h = h
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) and self._h.value:
+            capi.lib().mrg8_server_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):  # pragma: no cover
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_stream(self, stream_id: int, state, position: int = 0) -> None:
+        capi.call("mrg8_server_set_stream", self._h, ctypes.c_uint32(stream_id), _u32x8(state),
+                  ctypes.c_uint64(position))
+
+    def get_stream(self, stream_id: int):
+        st = (ctypes.c_uint32 * 8)()
+        pos = ctypes.c_uint64()
+        capi.call("mrg8_server_get_stream", self._h, ctypes.c_uint32(stream_id), st, ctypes.byref(pos))
+        return np.array(st[:], dtype=np.uint32), int(pos.value)
+
+    def submit(self, stream_id: int, count: int, fmt: int = F64) -> int:
+        rid = ctypes.c_uint64()
+        capi.call("mrg8_server_submit", self._h, ctypes.c_uint32(stream_id), ctypes.c_uint64(count), fmt,
+                  ctypes.byref(rid))
+        return int(rid.value)
+
+    def progress(self, max_done: int = 256, wait_ms: int = 0):
+        """Returns a list of completion dicts (request_id, stream_id, start, count, out, times)."""
+        buf = (capi.Completion * max(1, max_done))()
+        nd = ctypes.c_uint32()
+        capi.call("mrg8_server_progress", self._h, buf, ctypes.c_uint32(max_done), ctypes.byref(nd), wait_ms)
+        return [{"request_id": c.request_id, "stream_id": c.stream_id, "status": c.status, "start": c.start,
+                 "count": c.count, "out": c.out, "submit_time_s": c.submit_time_s,
+                 "complete_time_s": c.complete_time_s} for c in buf[:nd.value]]
+
+    def release(self, request_id: int) -> None:
+        capi.call("mrg8_server_release", self._h, ctypes.c_uint64(request_id))
+
+    def stats(self):
+        v = [ctypes.c_uint64() for _ in range(4)]
+        capi.call("mrg8_server_stats", self._h, *[ctypes.byref(x) for x in v])
+        return dict(zip(("queued", "inflight", "completed", "bytes_inflight"), [x.value for x in v]))
+
+    def replay(self, arrivals, streams, counts, fmt: int = F64, want_latencies: bool = False):
+        """Replays a trace (arrival seconds, stream ids, sizes); returns stats (and latencies)."""
+        n = len(arrivals)
+        tr = (capi.TraceEntry * max(1, n))()
+        for i in range(n):
+            tr[i].arrival_s = float(arrivals[i])
+            tr[i].stream_id = int(streams[i])
+            tr[i].format = fmt
+            tr[i].count = int(counts[i])
+        lat = np.zeros(max(1, n), dtype=np.float64)
+        st = capi.TraceStats()
+        capi.call("mrg8_server_replay", self._h, tr, ctypes.c_uint64(n),
+                  lat.ctypes.data_as(ctypes.POINTER(ctypes.c_double)) if want_latencies else None,
+                  ctypes.byref(st))
+        out = {k: getattr(st, k) for k, _ in capi.TraceStats._fields_}
+        return (out, lat[:n]) if want_latencies else out

@@ -39,6 +39,17 @@ class Completion(ctypes.Structure):
                 ("submit_time_s", ctypes.c_double), ("complete_time_s", ctypes.c_double)]
 
 
+class TraceEntry(ctypes.Structure):
+    _fields_ = [("arrival_s", ctypes.c_double), ("stream_id", ctypes.c_uint32), ("format", ctypes.c_int32),
+                ("count", ctypes.c_uint64)]
+
+
+class TraceStats(ctypes.Structure):
+    _fields_ = [("completed", ctypes.c_uint64), ("outputs", ctypes.c_uint64), ("wall_s", ctypes.c_double),
+                ("p50_s", ctypes.c_double), ("p90_s", ctypes.c_double), ("p99_s", ctypes.c_double),
+                ("max_s", ctypes.c_double), ("mean_s", ctypes.c_double)]
+
+
 u32p = ctypes.POINTER(ctypes.c_uint32)
 vp = ctypes.c_void_p
 H = ctypes.c_void_p  # opaque handles
@@ -93,6 +104,8 @@ SIGNATURES = {
     "mrg8_server_release": (ctypes.c_int, [H, ctypes.c_uint64]),
     "mrg8_server_stats": (ctypes.c_int, [H, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64),
                                          ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)]),
+    "mrg8_server_replay": (ctypes.c_int, [H, ctypes.POINTER(TraceEntry), ctypes.c_uint64,
+                                          ctypes.POINTER(ctypes.c_double), ctypes.POINTER(TraceStats)]),
 }
 
 _lib = None

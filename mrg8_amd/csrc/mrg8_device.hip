@@ -231,8 +231,10 @@ __global__ __launch_bounds__(1024) void k_batch(BatchArgs a) {
     const uint64_t first = local * a.tile;
     const uint64_t cnt = min(a.tile, sg.count - first);
     uint32_t s[8];
+    const uint32_t sid = __builtin_amdgcn_readfirstlane(sg.stream);
+    const uint32_t* src = sid == kInlineState ? sg.state : a.states + 8ull * sid;
 #pragma unroll
-    for (int j = 0; j < 8; ++j) s[j] = __builtin_amdgcn_readfirstlane(sg.state[j]);
+    for (int j = 0; j < 8; ++j) s[j] = __builtin_amdgcn_readfirstlane(src[j]);
     const uint64_t off = sg.offset + first;
     if (off) jump_uniform(s, off, a.jump);
     switch (__builtin_amdgcn_readfirstlane(sg.format)) {

@@ -35,15 +35,19 @@ struct FillArgs {
   const CoefRecord* jump;
 };
 
-// Work item of the batched kernel: `count` outputs of segment `desc`
-// starting `first` outputs into the segment.
+// A segment of the batched kernel: `count` outputs whose first one is
+// `offset` positions after the state of the segment's stream, written to
+// `out` in `format`.  The state is `state` itself when `stream` is
+// kInlineState, else word block `stream` of the resident state table.
+constexpr uint32_t kInlineState = 0xffffffffu;
+
 struct BatchSegment {
   uint32_t state[8];
-  uint64_t offset;   // position of output 0 of the segment relative to state
+  uint64_t offset;
   uint64_t count;
   void* out;
   int32_t format;
-  uint32_t pad;
+  uint32_t stream;
 };
 static_assert(sizeof(BatchSegment) == 64, "segment layout");
 
@@ -61,6 +65,7 @@ struct BatchArgs {
   const uint64_t* tile_begin;  // prefix sum of tiles per segment, nsegs + 1 entries
   uint64_t tile;               // outputs per tile
   uint32_t* cursor;            // work counter, zeroed at the start of the graph
+  const uint32_t* states;      // resident per-stream states (8 words each), may be null
   const CoefRecord* lane;
   const CoefRecord* jump;
 };

@@ -56,6 +56,11 @@ def lib() -> ctypes.CDLL:
         L.mrg8o_fnv1a.restype = ctypes.c_uint64
         L.mrg8o_hash_stream.argtypes = [u32p, ctypes.c_uint64, ctypes.c_int]
         L.mrg8o_hash_stream.restype = ctypes.c_uint64
+        L.mrg8o_wsum.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int]
+        L.mrg8o_wsum.restype = ctypes.c_uint64
+        L.mrg8o_stream_wsums.argtypes = [u32p, u32p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                         ctypes.c_uint64, ctypes.c_void_p]
+        L.mrg8o_stream_wsums.restype = None
         L.mrg8o_vec_outer_tp.argtypes = [u32p, u32p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int]
         L.mrg8o_vec_outer_tp.restype = ctypes.c_int
         L.mrg8o_vec_has_avx512.restype = ctypes.c_int
@@ -168,3 +173,21 @@ def vec_outer_tp(state, n: int, nthreads: int = 0, out: np.ndarray | None = None
 
 def vec_has_avx512() -> bool:
     return bool(lib().mrg8o_vec_has_avx512())
+
+
+def wsum(buf: np.ndarray) -> int:
+    """sum_i w_i (2i+1) mod 2^64 over the outputs (u32 words or f64 bit patterns)."""
+    b = np.ascontiguousarray(buf)
+    return int(lib().mrg8o_wsum(b.ctypes.data, b.size, b.itemsize))
+
+
+def stream_wsums(states, offsets, counts, formats) -> np.ndarray:
+    """wsum of each stream's outputs: state k jumped by offsets[k], counts[k] outputs."""
+    st = np.ascontiguousarray(np.asarray(states, dtype=np.uint32).reshape(-1, 8))
+    off = np.ascontiguousarray(np.asarray(offsets, dtype=np.uint64))
+    cnt = np.ascontiguousarray(np.asarray(counts, dtype=np.uint64))
+    fmt = np.ascontiguousarray(np.asarray(formats, dtype=np.int32))
+    out = np.zeros(len(cnt), dtype=np.uint64)
+    lib().mrg8o_stream_wsums(_ptr(jump_table()), _ptr(st), off.ctypes.data, cnt.ctypes.data, fmt.ctypes.data,
+                             len(cnt), out.ctypes.data)
+    return out

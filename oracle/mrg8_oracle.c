@@ -8,6 +8,7 @@
  */
 #include "mrg8_oracle.h"
 
+#include <omp.h>
 #include <stdio.h>
 #include <string.h>
 
@@ -207,4 +208,50 @@ uint64_t mrg8o_hash_stream(uint32_t st[8], uint64_t n, int format) {
     i += m;
   }
   return h;
+}
+
+uint64_t mrg8o_wsum(const void* buf, uint64_t n, int esz) {
+  uint64_t total = 0;
+#pragma omp parallel for reduction(+ : total) schedule(static)
+  for (int64_t i = 0; i < (int64_t)n; ++i) {
+    uint64_t w;
+    if (esz == 4)
+      w = ((const uint32_t*)buf)[i];
+    else
+      memcpy(&w, (const char*)buf + 8 * (size_t)i, 8);
+    total += w * (2 * (uint64_t)i + 1);
+  }
+  return total;
+}
+
+void mrg8o_stream_wsums(const uint32_t* tab, const uint32_t* states, const uint64_t* offsets,
+                        const uint64_t* counts, const int32_t* formats, uint64_t n, uint64_t* sums) {
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int64_t k = 0; k < (int64_t)n; ++k) {
+    uint32_t st[8];
+    mrg8o_jump_vec(tab, offsets[k], states + 8 * k, st);
+    enum { B = 4096 };
+    double d[B];
+    uint32_t u[B];
+    uint64_t total = 0, pos = 0;
+    while (pos < counts[k]) {
+      uint64_t m = counts[k] - pos < B ? counts[k] - pos : B;
+      if (formats[k] == 2) {
+        mrg8o_randint(st, u, m);
+        for (uint64_t i = 0; i < m; ++i) total += (uint64_t)u[i] * (2 * (pos + i) + 1);
+      } else {
+        if (formats[k] == 0)
+          mrg8o_rand(st, d, m);
+        else
+          mrg8o_rand01(st, d, m);
+        for (uint64_t i = 0; i < m; ++i) {
+          uint64_t w;
+          memcpy(&w, &d[i], 8);
+          total += w * (2 * (pos + i) + 1);
+        }
+      }
+      pos += m;
+    }
+    sums[k] = total;
+  }
 }

@@ -64,6 +64,16 @@ uint64_t mrg8o_fnv1a(uint64_t h, const void* buf, size_t nbytes);
  * advance st, and return the FNV-1a 64 hash of the little-endian bytes. */
 uint64_t mrg8o_hash_stream(uint32_t st[8], uint64_t n, int format);
 
+/* Position-weighted checksum sum_i w_i (2i+1) mod 2^64 over n outputs of
+ * esz bytes each (esz 4: words zero-extended; esz 8: IEEE bit patterns). */
+uint64_t mrg8o_wsum(const void* buf, uint64_t n, int esz);
+
+/* For each of n streams: jump states[8k..8k+7] by offsets[k], generate
+ * counts[k] outputs in formats[k], write their wsum to sums[k] (OpenMP over
+ * streams). */
+void mrg8o_stream_wsums(const uint32_t* tab, const uint32_t* states, const uint64_t* offsets,
+                        const uint64_t* counts, const int32_t* formats, uint64_t n, uint64_t* sums);
+
 /* CPU baselines (timed beside the GPU; reported, not optimised). */
 /* Vector + OpenMP restatement of knl/mrg8_vec.h mrg8_vec_outer_tp; writes
  * (x-1)/p doubles, advances st by n.  Returns the number of threads used. */

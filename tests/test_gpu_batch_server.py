@@ -1,0 +1,297 @@
+"""GPU parity of the batched launch and the serving loop against the oracle and
+the golden vectors of the compiled reference."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "ref_vectors.json")))
+DT = {0: np.float64, 1: np.float64, 2: np.uint32}
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+
+
+class Packed:
+    """Segments laid out back to back in one device buffer (element-aligned only)."""
+
+    def __init__(self, counts, fmts, gap_elems=1):
+        from mrg8_amd.hip import DeviceBuffer
+
+        self.offsets, pos = [], 0
+        for c, f in zip(counts, fmts):
+            esz = np.dtype(DT[f]).itemsize
+            pos = (pos + esz - 1) // esz * esz
+            self.offsets.append(pos)
+            pos += (int(c) + gap_elems) * esz
+        self.buf = DeviceBuffer(max(pos, 8))
+        self.buf.fill_byte(0xAB)
+        self.counts, self.fmts = counts, fmts
+
+    def ptr(self, i):
+        return self.buf.ptr + self.offsets[i]
+
+    def fetch(self, i):
+        return self.buf.copy_to_host(DT[self.fmts[i]], int(self.counts[i]), self.offsets[i])
+
+
+def test_batch_mixed_segments(gpu, oracle_mod):
+    rng = np.random.default_rng(21)
+    n = 200
+    states = rng.integers(0, 2**31, size=(n, 8)).astype(np.uint32)
+    states[::17, rng.integers(0, 8)] = 0
+    counts = rng.integers(1, 70000, size=n)
+    counts[::23] = 0
+    fmts = rng.integers(0, 3, size=n)
+    offs = [int(x) for x in rng.integers(0, 2**63, size=n, dtype=np.uint64)]
+    offs[:20] = list(range(20))  # small offsets take the shift rows of the jump
+    pk = Packed(counts, fmts, gap_elems=int(rng.integers(1, 4)))
+    b = gpu.Batch(max_streams=n, max_tiles=1 << 20)
+    b.set_config(256, 4, 4096)
+    segs = [(states[i], offs[i], int(counts[i]), pk.ptr(i), int(fmts[i])) for i in range(n)]
+    b.set(segs)
+    for rep in range(3):  # graph replays give identical results
+        say("batch-mixed", rep)
+        b.run()
+        gpu.hip.synchronize()
+        for i in range(n):
+            if counts[i] == 0:
+                continue
+            want, _ = oracle_mod.generate(oracle_mod.jump(states[i], offs[i]), int(counts[i]), int(fmts[i]))
+            got = pk.fetch(i)
+            assert np.array_equal(got.view(np.uint8), want.view(np.uint8)), (rep, i)
+        b.set_config(512, 2, 16384)  # a new shape, same outputs
+
+
+def test_batch_golden_set_state_streams(gpu, oracle_mod):
+    rows = GOLD["set_state_streams"]
+    counts = [r["n"] for r in rows]
+    fmts = [r["fmt"] for r in rows]
+    pk = Packed(counts, fmts)
+    b = gpu.Batch(max_streams=len(rows))
+    b.set([(r["init"], 0, r["n"], pk.ptr(i), r["fmt"]) for i, r in enumerate(rows)])
+    say("batch-golden")
+    b.run()
+    gpu.hip.synchronize()
+    for i, r in enumerate(rows):
+        assert f"{oracle_mod.fnv1a(pk.fetch(i)):016x}" == r["fnv1a64"], i
+
+
+def test_batch_golden_jump_then_randint(gpu, oracle_mod):
+    """The jump-heavy shape (63-bit steps from one seed, 256 words each), pinned
+    directly against the compiled reference's outputs."""
+    rows = GOLD["jump_then_randint"]
+    s0 = gpu.seed_state(GOLD["seed"])
+    pk = Packed([256] * len(rows), [2] * len(rows))
+    b = gpu.Batch(max_streams=len(rows))
+    b.set_config(256, 8, 256)
+    b.set([(s0, int(r["step"]), 256, pk.ptr(i), 2) for i, r in enumerate(rows)])
+    say("batch-jumps")
+    b.run()
+    gpu.hip.synchronize()
+    for i, r in enumerate(rows):
+        got = pk.fetch(i)
+        assert f"{oracle_mod.fnv1a(got):016x}" == r["fnv1a64"], i
+        assert [str(x) for x in got[:4]] == r["first"]
+
+
+def test_batch_autotune_keeps_outputs(gpu, oracle_mod, tmp_path):
+    rng = np.random.default_rng(5)
+    n = 64
+    states = rng.integers(1, 2**31 - 1, size=(n, 8)).astype(np.uint32)
+    counts = rng.integers(1000, 300000, size=n)
+    pk = Packed(counts, [0] * n)
+    b = gpu.Batch(max_streams=n)
+    b.set([(states[i], 0, int(counts[i]), pk.ptr(i), 0) for i in range(n)])
+    say("autotune")
+    cache = str(tmp_path / "tune.txt")
+    cfg = b.autotune(cache_path=cache)
+    assert cfg["tile_outputs"] in (16384, 65536, 262144)
+    assert os.path.exists(cache)
+    cfg2 = b.autotune(cache_path=cache)  # second call is served from the cache
+    assert cfg2 == cfg
+    pk.buf.fill_byte(0)
+    b.run()
+    gpu.hip.synchronize()
+    sums = oracle_mod.stream_wsums(states, np.zeros(n, dtype=np.uint64), counts, np.zeros(n, dtype=np.int32))
+    for i in range(n):
+        assert oracle_mod.wsum(pk.fetch(i)) == int(sums[i]), i
+
+
+def test_batch_errors(gpu):
+    from mrg8_amd.capi import Mrg8Error
+
+    b = gpu.Batch(max_streams=2, max_tiles=4)
+    with pytest.raises(Mrg8Error):
+        b.set([([1] * 8, 0, 10, 0, 0)])  # null output
+    with pytest.raises(Mrg8Error):
+        b.set([([1] * 8, 0, 10, 8, 7)])  # bad format
+    with pytest.raises(Mrg8Error):
+        b.set([([2**31] + [1] * 7, 0, 10, 8, 0)])  # word above p
+    with pytest.raises(Mrg8Error):
+        b.set([([1] * 8, 0, 10, 8, 0)] * 3)  # too many segments
+    with pytest.raises(Mrg8Error):
+        b.set([([1] * 8, 0, 10**9, 8, 0)])  # too many tiles
+    with pytest.raises(Mrg8Error):
+        b.set_config(100, 1, 1024)
+
+
+def test_c3_batched_full_size(gpu, oracle_mod):
+    """4096 streams of log-uniform length in [2^10, 2^22), float64, one launch;
+    every stream checked by its position-weighted checksum."""
+    from mrg8_amd import workloads
+    from mrg8_amd.hip import DeviceBuffer
+
+    states, lengths = workloads.batched()
+    total = int(lengths.sum())
+    say("c3", total)
+    buf = DeviceBuffer(total * 8)
+    starts = np.concatenate([[0], np.cumsum(lengths)[:-1]]).astype(np.uint64)
+    b = gpu.Batch(max_streams=len(lengths))
+    b.set([(states[i], 0, int(lengths[i]), buf.ptr + 8 * int(starts[i]), 0) for i in range(len(lengths))])
+    b.run()
+    gpu.hip.synchronize()
+    host = buf.copy_to_host(np.float64, total)
+    buf.free()
+    want = oracle_mod.stream_wsums(states, np.zeros(len(lengths), dtype=np.uint64), lengths,
+                                   np.zeros(len(lengths), dtype=np.int32))
+    bad = [i for i in range(len(lengths))
+           if oracle_mod.wsum(host[int(starts[i]):int(starts[i] + lengths[i])]) != int(want[i])]
+    assert not bad, bad[:10]
+
+
+# ---- serving -------------------------------------------------------------------------
+
+
+def drain(srv, expect, timeout_ms=20000):
+    got = {}
+    import time
+
+    t0 = time.time()
+    while len(got) < expect:
+        for c in srv.progress(max_done=256, wait_ms=50):
+            got[c["request_id"]] = c
+        assert time.time() - t0 < timeout_ms / 1000, "server did not finish"
+    return got
+
+
+def check_request(gpu, oracle_mod, c, base_state, fmt):
+    from mrg8_amd.hip import DeviceBuffer  # noqa: F401
+
+    dt = DT[fmt]
+    host = np.empty(int(c["count"]), dtype=dt)
+    from mrg8_amd import hip
+
+    hip.check(hip._load().hipMemcpy(host.ctypes.data, hip.ctypes.c_void_p(c["out"]), host.nbytes, 2), "D2H")
+    want, _ = oracle_mod.generate(oracle_mod.jump(base_state, int(c["start"])), int(c["count"]), fmt)
+    return np.array_equal(host.view(np.uint8), want.view(np.uint8)), host
+
+
+def test_server_requests_match_oracle(gpu, oracle_mod):
+    rng = np.random.default_rng(31)
+    srv = gpu.Server(num_streams=64, max_output_bytes=64 << 20, base_seed=1000, step_outputs=1 << 20,
+                     tile_outputs=8192)
+    reqs = {}
+    for k in range(300):
+        sid = int(rng.integers(0, 64))
+        cnt = int(rng.integers(1, 200000))
+        fmt = int(rng.integers(0, 3))
+        reqs[srv.submit(sid, cnt, fmt)] = (sid, fmt)
+    say("server-oracle")
+    done = drain(srv, len(reqs))
+    for rid, c in done.items():
+        sid, fmt = reqs[rid]
+        ok, _ = check_request(gpu, oracle_mod, c, oracle_mod.seed_state(1000 + sid), fmt)
+        assert ok, (rid, sid, c["start"], c["count"])
+        srv.release(rid)
+    st = srv.stats()
+    assert st["completed"] == 300 and st["bytes_inflight"] == 0 and st["queued"] == 0
+
+
+def test_server_alone_equals_mixed(gpu, oracle_mod):
+    """A request gives the same outputs alone and in the middle of a mix."""
+    seq = [(5, 1000, 0), (5, 5000, 2), (5, 77777, 0)]  # the third is the probe request
+    # alone: one request at a time
+    a = gpu.Server(num_streams=16, max_output_bytes=8 << 20, base_seed=7)
+    outs_alone = []
+    for sid, cnt, fmt in seq:
+        rid = a.submit(sid, cnt, fmt)
+        c = list(drain(a, 1).values())[0]
+        ok, host = check_request(gpu, oracle_mod, c, oracle_mod.seed_state(7 + sid), fmt)
+        assert ok
+        outs_alone.append(host)
+        a.release(rid)
+    # mixed: the same requests on stream 5, interleaved with 200 others on other streams
+    m = gpu.Server(num_streams=16, max_output_bytes=32 << 20, base_seed=7, step_outputs=1 << 18,
+                   tile_outputs=4096)
+    rng = np.random.default_rng(3)
+    mine = []
+    for k in range(200):
+        if k in (10, 80, 150):
+            sid, cnt, fmt = seq[len(mine)]
+            mine.append(m.submit(sid, cnt, fmt))
+        m.submit(int(rng.integers(6, 16)), int(rng.integers(1, 100000)), int(rng.integers(0, 3)))
+    say("server-mix")
+    done = drain(m, 203)
+    for j, rid in enumerate(mine):
+        sid, cnt, fmt = seq[j]
+        ok, host = check_request(gpu, oracle_mod, done[rid], oracle_mod.seed_state(7 + sid), fmt)
+        assert ok
+        assert np.array_equal(host.view(np.uint8), outs_alone[j].view(np.uint8))
+
+
+def test_server_admission_by_memory(gpu, oracle_mod):
+    cap = 4 << 20  # 4 MiB of output: at most four 1 MiB requests at once
+    srv = gpu.Server(num_streams=8, max_output_bytes=cap, base_seed=3, step_outputs=1 << 16, tile_outputs=4096)
+    ids = [srv.submit(i % 8, 131072, 0) for i in range(32)]  # 1 MiB each
+    seen_max = 0
+    done = {}
+    say("server-admission")
+    while len(done) < len(ids):
+        for c in srv.progress(max_done=4, wait_ms=20):
+            done[c["request_id"]] = c
+            ok, _ = check_request(gpu, oracle_mod, c, oracle_mod.seed_state(3 + c["stream_id"]), 0)
+            assert ok
+            srv.release(c["request_id"])
+        seen_max = max(seen_max, srv.stats()["bytes_inflight"])
+        assert srv.stats()["bytes_inflight"] <= cap
+    assert seen_max > 0
+
+
+def test_server_errors_and_cancel(gpu):
+    from mrg8_amd.capi import Mrg8Error
+
+    srv = gpu.Server(num_streams=4, max_output_bytes=1 << 20, base_seed=1)
+    with pytest.raises(Mrg8Error):
+        srv.submit(4, 10, 0)  # unknown stream
+    with pytest.raises(Mrg8Error):
+        srv.submit(0, 1 << 20, 0)  # larger than the cap
+    with pytest.raises(Mrg8Error):
+        srv.release(12345)
+    rid = srv.submit(1, 100, 2)  # queued: never progressed, so it can be cancelled
+    srv.release(rid)
+    st, pos = srv.get_stream(1)
+    assert pos == 100  # the cancelled request's positions stay consumed
+    srv.set_stream(1, [5] * 8, 0)
+    assert srv.get_stream(1)[0].tolist() == [5] * 8
+
+
+def test_server_replay_small_trace(gpu, oracle_mod):
+    srv = gpu.Server(num_streams=32, max_output_bytes=64 << 20, base_seed=11, step_outputs=1 << 20)
+    rng = np.random.default_rng(8)
+    n = 2000
+    arrivals = np.cumsum(rng.exponential(1e-4, size=n))
+    streams = rng.integers(0, 32, size=n)
+    sizes = np.floor(2.0 ** (8 + 10 * rng.random(n))).astype(np.uint64)
+    say("replay")
+    stats, lat = srv.replay(arrivals, streams, sizes, fmt=0, want_latencies=True)
+    assert stats["completed"] == n and stats["outputs"] == int(sizes.sum())
+    assert 0 < stats["p50_s"] <= stats["p99_s"] <= stats["max_s"]
+    assert (lat > 0).all()
+    # the replay consumed the streams' positions in trace order
+    for s in range(32):
+        assert srv.get_stream(s)[1] == int(sizes[streams == s].sum())
