@@ -191,7 +191,7 @@ def write_jump_matrix(path: str) -> None:
 class Batch:
     """Many stream segments generated by one persistent, graph-captured launch."""
 
-    def __init__(self, max_streams: int, max_tiles: int = 1 << 24, device: int = 0):
+    def __init__(self, max_streams: int, max_tiles: int = 1 << 21, device: int = 0):
         h = ctypes.c_void_p()
         capi.call("mrg8_batch_create", device, ctypes.c_uint32(max_streams), ctypes.c_uint64(max_tiles),
                   ctypes.byref(h))

@@ -45,6 +45,10 @@ LaunchConfig launch_config(const mrg8_batch_config& c) {
   return l;
 }
 
+uint64_t grid_waves(const mrg8_batch* b, const mrg8_batch_config& c) {
+  return (uint64_t)b->ctx->num_cu * (uint64_t)c.blocks_per_cu * (uint64_t)(c.block_threads / 64);
+}
+
 uint64_t count_tiles(const BatchSegment* segs, uint32_t n, uint64_t tile) {
   uint64_t t = 0;
   for (uint32_t i = 0; i < n; ++i) t += (segs[i].count + tile - 1) / tile;
@@ -66,7 +70,7 @@ extern "C" int mrg8_batch_create(int device, uint32_t max_streams, uint64_t max_
   if (!b) return fail(MRG8_ERR_NOMEM, "host allocation failed");
   b->ctx = ctx;
   b->max_tiles = max_tiles;
-  if ((rc = b->slot.init(ctx, max_streams, nullptr)) != MRG8_OK) {
+  if ((rc = b->slot.init(ctx, max_streams, max_tiles, nullptr)) != MRG8_OK) {
     delete b;
     return rc;
   }
@@ -94,12 +98,12 @@ extern "C" int mrg8_batch_set_config(mrg8_batch_t b, const mrg8_batch_config* cf
   int rc = validate_config(*cfg);
   if (rc) return rc;
   const uint32_t n = b->slot.staged_count();
-  if (cfg->tile_outputs != b->cfg.tile_outputs && n) {
-    // the tile prefix sums depend on the tile size
+  if (n) {
+    // the tile map and tiles per grab depend on the tile size and grid
     if (count_tiles(b->slot.segs(), n, cfg->tile_outputs) > b->max_tiles)
       return fail(MRG8_ERR_CAPACITY, "staged segments need more than max_tiles tiles at this tile size");
     if ((rc = b->slot.wait_staging()) != MRG8_OK) return rc;
-    b->slot.finalise(n, cfg->tile_outputs);
+    b->slot.finalise(n, cfg->tile_outputs, grid_waves(b, *cfg));
   }
   b->cfg = *cfg;
   return MRG8_OK;
@@ -141,7 +145,7 @@ extern "C" int mrg8_batch_set(mrg8_batch_t b, const mrg8_stream_desc* descs, uin
     s.format = d.format;
     s.stream = kInlineState;
   }
-  b->slot.finalise(count, b->cfg.tile_outputs);
+  b->slot.finalise(count, b->cfg.tile_outputs, grid_waves(b, b->cfg));
   b->total = total;
   return MRG8_OK;
 }

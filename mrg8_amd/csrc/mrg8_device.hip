@@ -194,8 +194,8 @@ __global__ __launch_bounds__(1024) void k_fill(FillArgs a) {
   generate_run<FMT, PL>(static_cast<typename Out<FMT>::T*>(a.out) + first, cnt, s, a.lane);
 }
 
-// Persistent batched kernel: waves take tiles from a global counter until the
-// list is exhausted (every wave exits once the counter passes ntiles).
+// One tile of a batch segment: picks the store width from the destination's
+// alignment (segments may start at any element).
 template <int FMT>
 __device__ __forceinline__ void batch_tile(const BatchSegment& sg, uint64_t first, uint64_t cnt, uint32_t s[8],
                                            const CoefRecord* __restrict__ lane_rec) {
@@ -212,35 +212,77 @@ __device__ __forceinline__ void batch_tile(const BatchSegment& sg, uint64_t firs
   }
 }
 
+// Advances eight states at once: lane group g = lane / 8 holds its own state
+// (replicated in the group's lanes, s[] per lane) and its own 64-bit offset;
+// lane r of a group computes row r of A^(2^b) for the group's lowest set bit
+// b, and the group's eight new words are gathered back with ds_bpermute.
+// Groups with no bits left idle until every group is done.
+__device__ void jump_groups(uint32_t s[8], uint64_t off, const CoefRecord* __restrict__ jrec) {
+  const int lane = threadIdx.x & 63;
+  const int grp = lane >> 3, row = lane & 7;
+  Coef k = {};
+  if (off) k = load_coef(jrec + (__builtin_ctzll(off) * 8 + row));
+  while (__ballot(off != 0)) {
+    const bool active = off != 0;
+    const uint64_t rest = active ? off & (off - 1) : 0;
+    Coef next = {};
+    if (rest) next = load_coef(jrec + (__builtin_ctzll(rest) * 8 + row));  // one step ahead
+    int32_t e[8];
+    uint32_t nz = 0;
+    centre(s, e);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) nz |= (s[j] != 0u ? 1u : 0u) << j;
+    const uint32_t y = (k.support & nz) ? dot_residue(k, e) + 1u : 0u;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const uint32_t w = (uint32_t)__builtin_amdgcn_ds_bpermute((grp * 8 + j) * 4, (int)y);
+      s[j] = active ? w : s[j];
+    }
+    off = rest;
+    k = next;
+  }
+}
+
+// Persistent batched kernel.  A wave takes `grab` consecutive tiles per
+// counter increment, jumps their states together (jump_groups), then
+// generates the tiles one after another.  Every wave exits once the counter
+// passes ntiles.
 __global__ __launch_bounds__(1024) void k_batch(BatchArgs a) {
   const int lane = threadIdx.x & 63;
+  const int grp = lane >> 3;
+  const uint64_t ntiles = a.hdr->ntiles;
+  const uint32_t grab = a.hdr->grab;
   for (;;) {
-    uint32_t t = 0;
-    if (lane == 0) t = atomicAdd(a.cursor, 1u);
-    const uint64_t tile = (uint64_t)__builtin_amdgcn_readfirstlane(t);
-    if (tile >= a.hdr->ntiles) return;
-    // segment holding this tile: last i with tile_begin[i] <= tile (empty
-    // segments share their begin with the next one and are skipped)
-    uint32_t lo = 0, hi = a.hdr->nsegs;
-    while (hi - lo > 1) {
-      const uint32_t mid = (lo + hi) >> 1;
-      if (a.tile_begin[mid] <= tile) lo = mid; else hi = mid;
-    }
-    const BatchSegment sg = a.segs[lo];
-    const uint64_t local = tile - a.tile_begin[lo];
-    const uint64_t first = local * a.tile;
-    const uint64_t cnt = min(a.tile, sg.count - first);
+    uint32_t t0 = 0;
+    if (lane == 0) t0 = atomicAdd(a.cursor, grab);
+    const uint64_t base = (uint64_t)__builtin_amdgcn_readfirstlane(t0);
+    if (base >= ntiles) return;
+    const uint32_t ngrp = (uint32_t)min<uint64_t>(grab, ntiles - base);
+    // per group: its tile, segment, starting state and offset
+    const bool mine = (uint32_t)grp < ngrp;
+    const uint64_t t = base + (mine ? grp : 0);
+    const uint32_t seg = a.tile_seg[t];
+    const BatchSegment* sp = a.segs + seg;
+    const uint64_t first = (t - a.tile_begin[seg]) * a.tile;
+    const uint32_t sid = sp->stream;
+    const uint32_t* src = sid == kInlineState ? sp->state : a.states + 8ull * sid;
     uint32_t s[8];
-    const uint32_t sid = __builtin_amdgcn_readfirstlane(sg.stream);
-    const uint32_t* src = sid == kInlineState ? sg.state : a.states + 8ull * sid;
 #pragma unroll
-    for (int j = 0; j < 8; ++j) s[j] = __builtin_amdgcn_readfirstlane(src[j]);
-    const uint64_t off = sg.offset + first;
-    if (off) jump_uniform(s, off, a.jump);
-    switch (__builtin_amdgcn_readfirstlane(sg.format)) {
-      case 0: batch_tile<0>(sg, first, cnt, s, a.lane); break;
-      case 1: batch_tile<1>(sg, first, cnt, s, a.lane); break;
-      default: batch_tile<2>(sg, first, cnt, s, a.lane); break;
+    for (int j = 0; j < 8; ++j) s[j] = src[j];
+    jump_groups(s, mine ? sp->offset + first : 0, a.jump);
+    for (uint32_t g = 0; g < ngrp; ++g) {
+      uint32_t u[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) u[j] = __builtin_amdgcn_readlane(s[j], 8 * g);
+      const uint32_t useg = __builtin_amdgcn_readlane(seg, 8 * g);
+      const BatchSegment sg = a.segs[useg];
+      const uint64_t ufirst = ((base + g) - a.tile_begin[useg]) * a.tile;
+      const uint64_t cnt = min(a.tile, sg.count - ufirst);
+      switch (__builtin_amdgcn_readfirstlane(sg.format)) {
+        case 0: batch_tile<0>(sg, ufirst, cnt, u, a.lane); break;
+        case 1: batch_tile<1>(sg, ufirst, cnt, u, a.lane); break;
+        default: batch_tile<2>(sg, ufirst, cnt, u, a.lane); break;
+      }
     }
   }
 }

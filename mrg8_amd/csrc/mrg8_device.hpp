@@ -55,7 +55,7 @@ static_assert(sizeof(BatchSegment) == 64, "segment layout");
 // re-capturing the launch.
 struct BatchHeader {
   uint32_t nsegs;
-  uint32_t pad;
+  uint32_t grab;     // tiles a wave takes per counter increment (1..8)
   uint64_t ntiles;
 };
 
@@ -63,6 +63,7 @@ struct BatchArgs {
   const BatchHeader* hdr;
   const BatchSegment* segs;
   const uint64_t* tile_begin;  // prefix sum of tiles per segment, nsegs + 1 entries
+  const uint32_t* tile_seg;    // segment of each tile, ntiles entries
   uint64_t tile;               // outputs per tile
   uint32_t* cursor;            // work counter, zeroed at the start of the graph
   const uint32_t* states;      // resident per-stream states (8 words each), may be null

@@ -210,7 +210,8 @@ int launch_steps(mrg8_server* s) {
       }
     }
     if (n == 0) return MRG8_OK;
-    step->slot.finalise(n, s->launch.tile);
+    step->slot.finalise(n, s->launch.tile,
+                        (uint64_t)s->ctx->num_cu * s->launch.blocks_per_cu * (s->launch.block_threads / 64));
     if ((rc = step->slot.run(s->launch, s->stream)) != MRG8_OK) return rc;
     step->busy = true;
     s->inflight.push_back(step);
@@ -270,9 +271,10 @@ extern "C" int mrg8_server_create(int device, const mrg8_server_config* cfg, uin
   s->arena.init(s->cfg.max_output_bytes);
   // Segments per step: every admitted request may contribute one chunk.
   const uint32_t seg_cap = std::min<uint32_t>(s->cfg.max_inflight, 1u << 16);
+  const uint64_t tile_cap = s->cfg.step_outputs / s->cfg.tile_outputs + seg_cap + 1;
   for (int i = 0; i < kSlots; ++i) {
     auto st = std::make_unique<Step>();
-    if ((rc = st->slot.init(ctx, seg_cap, s->d_states)) != MRG8_OK) {
+    if ((rc = st->slot.init(ctx, seg_cap, tile_cap, s->d_states)) != MRG8_OK) {
       mrg8_server_destroy(s.release());
       return rc;
     }

@@ -10,22 +10,24 @@ namespace {
 size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
 struct Layout {
-  size_t segs, tiles, cursor, total;
-  explicit Layout(uint32_t cap) {
+  size_t segs, tiles, tile_seg, cursor, total;
+  Layout(uint32_t cap, uint64_t max_tiles) {
     segs = align_up(sizeof(BatchHeader), 64);
     tiles = segs + sizeof(BatchSegment) * (size_t)cap;
-    cursor = align_up(tiles + sizeof(uint64_t) * ((size_t)cap + 1), 256);
+    tile_seg = align_up(tiles + sizeof(uint64_t) * ((size_t)cap + 1), 64);
+    cursor = align_up(tile_seg + sizeof(uint32_t) * (size_t)max_tiles, 256);
     total = cursor + 256;
   }
 };
 }  // namespace
 
-int BatchSlot::init(DeviceContext* ctx, uint32_t cap, const uint32_t* states) {
+int BatchSlot::init(DeviceContext* ctx, uint32_t cap, uint64_t max_tiles, const uint32_t* states) {
   release();
   ctx_ = ctx;
   cap_ = cap;
+  max_tiles_ = max_tiles;
   states_ = states;
-  const Layout l(cap);
+  const Layout l(cap, max_tiles);
   DeviceGuard dg(ctx->device);
   hipError_t e = hipMalloc((void**)&dmem_, l.total);
   if (e == hipSuccess) e = hipHostMalloc((void**)&hmem_, l.cursor, hipHostMallocDefault);
@@ -40,10 +42,12 @@ int BatchSlot::init(DeviceContext* ctx, uint32_t cap, const uint32_t* states) {
   d_hdr_ = reinterpret_cast<BatchHeader*>(dmem_);
   d_segs_ = reinterpret_cast<BatchSegment*>(dmem_ + l.segs);
   d_tiles_ = reinterpret_cast<uint64_t*>(dmem_ + l.tiles);
+  d_tile_seg_ = reinterpret_cast<uint32_t*>(dmem_ + l.tile_seg);
   d_cursor_ = reinterpret_cast<uint32_t*>(dmem_ + l.cursor);
   h_hdr_ = reinterpret_cast<BatchHeader*>(hmem_);
   h_segs_ = reinterpret_cast<BatchSegment*>(hmem_ + l.segs);
   h_tiles_ = reinterpret_cast<uint64_t*>(hmem_ + l.tiles);
+  h_tile_seg_ = reinterpret_cast<uint32_t*>(hmem_ + l.tile_seg);
   return MRG8_OK;
 }
 
@@ -75,18 +79,25 @@ int BatchSlot::wait_staging() {
   return MRG8_OK;
 }
 
-uint64_t BatchSlot::finalise(uint32_t count, uint64_t tile) {
+uint64_t BatchSlot::finalise(uint32_t count, uint64_t tile, uint64_t grid_waves) {
   uint64_t t = 0;
   for (uint32_t i = 0; i < count; ++i) {
     h_tiles_[i] = t;
-    t += (h_segs_[i].count + tile - 1) / tile;
+    const uint64_t n = (h_segs_[i].count + tile - 1) / tile;
+    for (uint64_t k = 0; k < n && t + k < max_tiles_; ++k) h_tile_seg_[t + k] = i;
+    t += n;
   }
   h_tiles_[count] = t;
+  // Take several tiles per counter increment only when there are enough
+  // tiles to keep every wave busy (their jumps then share one pass).
+  uint64_t grab = grid_waves ? t / (grid_waves * 4) : 1;
+  grab = grab < 1 ? 1 : (grab > 8 ? 8 : grab);
   h_hdr_->nsegs = count;
-  h_hdr_->pad = 0;
+  h_hdr_->grab = (uint32_t)grab;
   h_hdr_->ntiles = t;
   count_ = count;
   ntiles_ = t;
+  dirty_ = true;
   return t;
 }
 
@@ -107,6 +118,7 @@ int BatchSlot::capture(const LaunchConfig& cfg) {
   a.hdr = d_hdr_;
   a.segs = d_segs_;
   a.tile_begin = d_tiles_;
+  a.tile_seg = d_tile_seg_;
   a.tile = cfg.tile;
   a.cursor = d_cursor_;
   a.states = states_;
@@ -130,17 +142,24 @@ int BatchSlot::capture(const LaunchConfig& cfg) {
 
 int BatchSlot::run(const LaunchConfig& cfg, hipStream_t stream) {
   if (count_ == 0 || ntiles_ == 0) return MRG8_OK;
+  if (ntiles_ > max_tiles_) return fail(MRG8_ERR_CAPACITY, "staged work needs more than max_tiles tiles");
   DeviceGuard dg(ctx_->device);
   int rc = capture(cfg);
   if (rc) return rc;
-  hipError_t e = hipMemcpyAsync(d_hdr_, h_hdr_, sizeof(BatchHeader), hipMemcpyHostToDevice, stream);
-  if (e == hipSuccess)
-    e = hipMemcpyAsync(d_segs_, h_segs_, sizeof(BatchSegment) * count_, hipMemcpyHostToDevice, stream);
-  if (e == hipSuccess)
-    e = hipMemcpyAsync(d_tiles_, h_tiles_, sizeof(uint64_t) * (count_ + 1), hipMemcpyHostToDevice, stream);
-  if (e == hipSuccess) e = hipEventRecord(staged_, stream);
-  if (e != hipSuccess) return hip_fail(e, "batch staging copy");
-  copy_pending_ = true;
+  hipError_t e = hipSuccess;
+  if (dirty_) {  // replays of an unchanged batch copy nothing
+    e = hipMemcpyAsync(d_hdr_, h_hdr_, sizeof(BatchHeader), hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(d_segs_, h_segs_, sizeof(BatchSegment) * count_, hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(d_tiles_, h_tiles_, sizeof(uint64_t) * (count_ + 1), hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(d_tile_seg_, h_tile_seg_, sizeof(uint32_t) * ntiles_, hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess) e = hipEventRecord(staged_, stream);
+    if (e != hipSuccess) return hip_fail(e, "batch staging copy");
+    copy_pending_ = true;
+    dirty_ = false;
+  }
   e = hipGraphLaunch(exec_, stream);
   if (e == hipSuccess) e = hipEventRecord(run_done_, stream);
   if (e != hipSuccess) return hip_fail(e, "hipGraphLaunch");

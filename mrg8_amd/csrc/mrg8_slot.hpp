@@ -1,11 +1,13 @@
 // A batch slot: everything one in-flight persistent launch needs.
 //
 // Device block (one allocation):
-//   BatchHeader | BatchSegment[cap] | uint64 tile_begin[cap+1] | uint32 cursor
-// Host side: a pinned staging copy of the same prefix.  A run copies the
-// used prefix host-to-device and replays a captured graph holding
-// [zero the cursor] -> [persistent kernel]; the graph reads the counts from
-// the device header, so it is captured once per launch configuration.
+//   BatchHeader | BatchSegment[cap] | uint64 tile_begin[cap+1] |
+//   uint32 tile_seg[max_tiles] | uint32 cursor
+// Host side: a pinned staging copy of everything before the cursor.  A run
+// copies the used parts host-to-device when they changed since the last run
+// and replays a captured graph holding [zero the cursor] -> [persistent
+// kernel]; the graph reads the counts from the device header, so it is
+// captured once per launch configuration.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -29,19 +31,21 @@ class BatchSlot {
   BatchSlot& operator=(const BatchSlot&) = delete;
   ~BatchSlot() { release(); }
 
-  // Allocates room for `cap` segments; `states` is the resident state table
-  // the segments may refer to (may be null).
-  int init(DeviceContext* ctx, uint32_t cap, const uint32_t* states);
+  // Allocates room for `cap` segments and `max_tiles` tiles; `states` is the
+  // resident state table the segments may refer to (may be null).
+  int init(DeviceContext* ctx, uint32_t cap, uint64_t max_tiles, const uint32_t* states);
   void release();
 
   // Host staging (valid to write only when idle()).
   BatchSegment* segs() { return h_segs_; }
   uint32_t capacity() const { return cap_; }
+  uint64_t max_tiles() const { return max_tiles_; }
   // Waits until the previous run no longer reads the staging block.
   int wait_staging();
-  // Finalises `count` staged segments (computes tile prefix sums for `tile`).
-  // Returns the number of tiles.
-  uint64_t finalise(uint32_t count, uint64_t tile);
+  // Finalises `count` staged segments for tiles of `tile` outputs and a grid
+  // of `grid_waves` waves (prefix sums, tile -> segment map, tiles per
+  // grab).  Returns the number of tiles; the caller keeps it <= max_tiles.
+  uint64_t finalise(uint32_t count, uint64_t tile, uint64_t grid_waves);
   uint64_t staged_tiles() const { return ntiles_; }
   uint32_t staged_count() const { return count_; }
 
@@ -57,18 +61,21 @@ class BatchSlot {
 
   DeviceContext* ctx_ = nullptr;
   uint32_t cap_ = 0;
+  uint64_t max_tiles_ = 0;
   const uint32_t* states_ = nullptr;
   char* dmem_ = nullptr;
   char* hmem_ = nullptr;
   BatchHeader* d_hdr_ = nullptr;
   BatchSegment* d_segs_ = nullptr;
   uint64_t* d_tiles_ = nullptr;
+  uint32_t* d_tile_seg_ = nullptr;
   uint32_t* d_cursor_ = nullptr;
   BatchHeader* h_hdr_ = nullptr;
   BatchSegment* h_segs_ = nullptr;
   uint64_t* h_tiles_ = nullptr;
+  uint32_t* h_tile_seg_ = nullptr;
   hipEvent_t staged_ = nullptr, run_done_ = nullptr;
-  bool copy_pending_ = false, run_pending_ = false;
+  bool copy_pending_ = false, run_pending_ = false, dirty_ = false;
   uint32_t count_ = 0;
   uint64_t ntiles_ = 0;
   hipGraph_t graph_ = nullptr;

@@ -1,0 +1,79 @@
+"""Fixed workload for rocprofv3 runs (kernel traces and HBM counters).
+
+    rocprofv3 --kernel-trace --stats -d OUT -o run -- python3 -m mrg8_amd.profile_workload
+    rocprofv3 --pmc FETCH_SIZE -d OUT -o fetch -- python3 -m mrg8_amd.profile_workload
+    rocprofv3 --pmc WRITE_SIZE -d OUT -o write -- python3 -m mrg8_amd.profile_workload
+
+Dispatches, in order (each `--reps` times):
+  1. a runtime device-to-device copy of 1 GiB  -- the calibration kernel of
+     known traffic (1 GiB read + 1 GiB written)
+  2. single-stream fill, 2^28 float64 outputs (seed 13579)
+  3. single-stream fill, 2^28 uint32 outputs
+  4. batched launch: 4096 streams, log-uniform lengths, float64   (--batched)
+  5. jump-heavy batched launch: 65536 x (63-bit jump + 256 uint32) (--batched)
+"""
+from __future__ import annotations
+
+import argparse
+import os
+import sys
+
+import numpy as np
+
+sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--reps", type=int, default=3)
+    ap.add_argument("--batched", action="store_true")
+    args = ap.parse_args()
+
+    import mrg8_amd
+    from mrg8_amd import hip, workloads
+
+    n = 1 << 28
+    src = hip.DeviceBuffer(1 << 30)
+    dst = hip.DeviceBuffer(1 << 30)
+    src.fill_byte(1)
+    for _ in range(args.reps):
+        hip.check(hip._load().hipMemcpy(hip.ctypes.c_void_p(dst.ptr), hip.ctypes.c_void_p(src.ptr), 1 << 30, 3),
+                  "hipMemcpy D2D")
+    hip.synchronize()
+    src.free()
+    dst.free()
+    buf = hip.DeviceBuffer(n * 8)
+    g = mrg8_amd.mrg8_cuda(workloads.SEED)
+    for fmt in (mrg8_amd.F64, mrg8_amd.U32):
+        for _ in range(args.reps):
+            g.generate(buf, n, fmt)
+        g.synchronize()
+    buf.free()
+    if args.batched:
+        states, lengths = workloads.batched()
+        total = int(lengths.sum())
+        out = hip.DeviceBuffer(total * 8)
+        starts = np.concatenate([[0], np.cumsum(lengths)[:-1]]).astype(np.uint64)
+        b = mrg8_amd.Batch(max_streams=len(lengths))
+        b.set([(states[i], 0, int(lengths[i]), out.ptr + 8 * int(starts[i]), mrg8_amd.F64)
+               for i in range(len(lengths))])
+        for _ in range(args.reps):
+            b.run()
+        hip.synchronize()
+        out.free()
+        steps = workloads.jump_steps()
+        m = len(steps)
+        out = hip.DeviceBuffer(m * 1024)
+        s0 = mrg8_amd.seed_state(workloads.SEED)
+        j = mrg8_amd.Batch(max_streams=m, max_tiles=m)
+        j.set_config(256, 8, 256)
+        j.set([(s0, int(steps[i]), 256, out.ptr + 1024 * i, mrg8_amd.U32) for i in range(m)])
+        for _ in range(args.reps):
+            j.run()
+        hip.synchronize()
+        out.free()
+    print("profile workload done", flush=True)
+
+
+if __name__ == "__main__":
+    main()

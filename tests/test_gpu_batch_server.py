@@ -164,28 +164,59 @@ def test_c3_batched_full_size(gpu, oracle_mod):
     assert not bad, bad[:10]
 
 
+def test_c4_jump_heavy_full_size(gpu, oracle_mod):
+    """65536 streams from one seed, each jumped by a uniform step in [0, 2^63),
+    then 256 words: every stream against the oracle's jump + generation."""
+    from mrg8_amd import workloads
+    from mrg8_amd.hip import DeviceBuffer
+
+    steps = workloads.jump_steps()
+    n = len(steps)
+    s0 = gpu.seed_state(workloads.SEED)
+    buf = DeviceBuffer(n * 1024)
+    b = gpu.Batch(max_streams=n, max_tiles=n)
+    b.set_config(256, 8, 256)
+    b.set([(s0, int(steps[i]), 256, buf.ptr + 1024 * i, 2) for i in range(n)])
+    say("c4", n)
+    b.run()
+    gpu.hip.synchronize()
+    host = buf.copy_to_host(np.uint32, n * 256).reshape(n, 256)
+    buf.free()
+    want = oracle_mod.stream_wsums(np.tile(s0, (n, 1)), steps, np.full(n, 256, dtype=np.uint64),
+                                   np.full(n, 2, dtype=np.int32))
+    weights = np.arange(256, dtype=np.uint64) * np.uint64(2) + np.uint64(1)
+    got = np.sum(host.astype(np.uint64) * weights, axis=1, dtype=np.uint64)
+    bad = np.flatnonzero(got != want).tolist()
+    assert not bad, bad[:10]
+    # spot-check full words for a few streams
+    for i in (0, 1, n // 2, n - 1):
+        w, _ = oracle_mod.generate(oracle_mod.jump(s0, int(steps[i])), 256, 2)
+        assert np.array_equal(host[i], w)
+
+
 # ---- serving -------------------------------------------------------------------------
 
 
-def drain(srv, expect, timeout_ms=20000):
-    got = {}
+def drain(srv, expect, on_done, timeout_s=60):
+    """Progress the server until `expect` requests completed; each completion is
+    passed to on_done and then released (admission is by output memory, so a
+    caller that never releases stalls the server by design)."""
     import time
 
-    t0 = time.time()
-    while len(got) < expect:
+    t0, n = time.time(), 0
+    while n < expect:
         for c in srv.progress(max_done=256, wait_ms=50):
-            got[c["request_id"]] = c
-        assert time.time() - t0 < timeout_ms / 1000, "server did not finish"
-    return got
+            on_done(c)
+            srv.release(c["request_id"])
+            n += 1
+        assert time.time() - t0 < timeout_s, f"server stalled after {n}/{expect} completions: {srv.stats()}"
 
 
 def check_request(gpu, oracle_mod, c, base_state, fmt):
-    from mrg8_amd.hip import DeviceBuffer  # noqa: F401
+    from mrg8_amd import hip
 
     dt = DT[fmt]
     host = np.empty(int(c["count"]), dtype=dt)
-    from mrg8_amd import hip
-
     hip.check(hip._load().hipMemcpy(host.ctypes.data, hip.ctypes.c_void_p(c["out"]), host.nbytes, 2), "D2H")
     want, _ = oracle_mod.generate(oracle_mod.jump(base_state, int(c["start"])), int(c["count"]), fmt)
     return np.array_equal(host.view(np.uint8), want.view(np.uint8)), host
@@ -202,12 +233,13 @@ def test_server_requests_match_oracle(gpu, oracle_mod):
         fmt = int(rng.integers(0, 3))
         reqs[srv.submit(sid, cnt, fmt)] = (sid, fmt)
     say("server-oracle")
-    done = drain(srv, len(reqs))
-    for rid, c in done.items():
-        sid, fmt = reqs[rid]
+
+    def check(c):
+        sid, fmt = reqs[c["request_id"]]
         ok, _ = check_request(gpu, oracle_mod, c, oracle_mod.seed_state(1000 + sid), fmt)
-        assert ok, (rid, sid, c["start"], c["count"])
-        srv.release(rid)
+        assert ok, (c["request_id"], sid, c["start"], c["count"])
+
+    drain(srv, len(reqs), check)
     st = srv.stats()
     assert st["completed"] == 300 and st["bytes_inflight"] == 0 and st["queued"] == 0
 
@@ -219,29 +251,38 @@ def test_server_alone_equals_mixed(gpu, oracle_mod):
     a = gpu.Server(num_streams=16, max_output_bytes=8 << 20, base_seed=7)
     outs_alone = []
     for sid, cnt, fmt in seq:
-        rid = a.submit(sid, cnt, fmt)
-        c = list(drain(a, 1).values())[0]
-        ok, host = check_request(gpu, oracle_mod, c, oracle_mod.seed_state(7 + sid), fmt)
-        assert ok
-        outs_alone.append(host)
-        a.release(rid)
+        a.submit(sid, cnt, fmt)
+
+        def keep(c, sid=sid, fmt=fmt):
+            ok, host = check_request(gpu, oracle_mod, c, oracle_mod.seed_state(7 + sid), fmt)
+            assert ok
+            outs_alone.append(host)
+
+        drain(a, 1, keep)
     # mixed: the same requests on stream 5, interleaved with 200 others on other streams
     m = gpu.Server(num_streams=16, max_output_bytes=32 << 20, base_seed=7, step_outputs=1 << 18,
                    tile_outputs=4096)
     rng = np.random.default_rng(3)
-    mine = []
+    mine = {}
     for k in range(200):
         if k in (10, 80, 150):
             sid, cnt, fmt = seq[len(mine)]
-            mine.append(m.submit(sid, cnt, fmt))
+            mine[m.submit(sid, cnt, fmt)] = len(mine)
         m.submit(int(rng.integers(6, 16)), int(rng.integers(1, 100000)), int(rng.integers(0, 3)))
     say("server-mix")
-    done = drain(m, 203)
-    for j, rid in enumerate(mine):
-        sid, cnt, fmt = seq[j]
-        ok, host = check_request(gpu, oracle_mod, done[rid], oracle_mod.seed_state(7 + sid), fmt)
-        assert ok
-        assert np.array_equal(host.view(np.uint8), outs_alone[j].view(np.uint8))
+    outs_mixed = {}
+
+    def grab(c):
+        if c["request_id"] in mine:
+            j = mine[c["request_id"]]
+            sid, cnt, fmt = seq[j]
+            ok, host = check_request(gpu, oracle_mod, c, oracle_mod.seed_state(7 + sid), fmt)
+            assert ok
+            outs_mixed[j] = host
+
+    drain(m, 203, grab)
+    for j in range(3):
+        assert np.array_equal(outs_mixed[j].view(np.uint8), outs_alone[j].view(np.uint8)), j
 
 
 def test_server_admission_by_memory(gpu, oracle_mod):
@@ -252,14 +293,18 @@ def test_server_admission_by_memory(gpu, oracle_mod):
     done = {}
     say("server-admission")
     while len(done) < len(ids):
-        for c in srv.progress(max_done=4, wait_ms=20):
+        got = srv.progress(max_done=4, wait_ms=20)
+        st = srv.stats()  # completed-but-unreleased requests still hold their bytes
+        seen_max = max(seen_max, st["bytes_inflight"])
+        assert st["bytes_inflight"] <= cap
+        assert st["inflight"] + len(got) <= 4  # memory admits at most four at a time
+        for c in got:
             done[c["request_id"]] = c
             ok, _ = check_request(gpu, oracle_mod, c, oracle_mod.seed_state(3 + c["stream_id"]), 0)
             assert ok
             srv.release(c["request_id"])
-        seen_max = max(seen_max, srv.stats()["bytes_inflight"])
-        assert srv.stats()["bytes_inflight"] <= cap
-    assert seen_max > 0
+    assert seen_max >= 1 << 20
+    assert srv.stats()["bytes_inflight"] == 0
 
 
 def test_server_errors_and_cancel(gpu):
