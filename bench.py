@@ -49,20 +49,19 @@ def log(*a):
 
 
 def time_launches(fn, stream, reps):
-    """Mean ms per launch from HIP events recorded around each launch on `stream`."""
+    """Mean ms per launch: HIP events on `stream` bracketing `reps` back-to-back launches."""
     from mrg8_amd import hip
 
-    evs = [(hip.Event(), hip.Event()) for _ in range(reps)]
-    for a, b in evs:
-        a.record(stream)
+    e0, e1 = hip.Event(), hip.Event()
+    e0.record(stream)
+    for _ in range(reps):
         fn()
-        b.record(stream)
-    evs[-1][1].synchronize()
-    ms = [a.elapsed_ms(b) for a, b in evs]
-    for a, b in evs:
-        a.destroy()
-        b.destroy()
-    return float(np.mean(ms)), float(np.min(ms))
+    e1.record(stream)
+    e1.synchronize()
+    ms = e0.elapsed_ms(e1) / reps
+    e0.destroy()
+    e1.destroy()
+    return ms
 
 
 def traffic_from_profiles(name):
@@ -103,16 +102,16 @@ def bench_single(args):
         hip.synchronize()
         wall = time.perf_counter() - t0
         ev_s = e0.elapsed_ms(e1) / 1e3
-        kern_ms, kern_min = time_launches(step, s, max(3, min(args.steps, 10)))
-        out[name] = {"wall_s": wall, "event_s": ev_s, "kernel_ms": kern_ms, "kernel_ms_min": kern_min,
-                     "bytes": N_SINGLE * esz}
-        log(f"single {name}: {N_SINGLE * args.steps / wall / 1e9:.1f} G samples/s wall, kernel {kern_ms:.3f} ms")
+        # mean launch duration: HIP events bracketing the timed region on the launch stream
+        out[name] = {"wall_s": wall, "event_s": ev_s, "kernel_ms": ev_s * 1e3 / args.steps, "bytes": N_SINGLE * esz}
+        log(f"single {name}: {N_SINGLE * args.steps / wall / 1e9:.1f} G samples/s wall, "
+            f"kernel {out[name]['kernel_ms']:.3f} ms")
     # the reference's full-compare size
     small = 1 << 24
     step = lambda: g.generate(buf, small, mrg8_amd.F64, stream=s)  # noqa: E731
     for _ in range(3):
         step()
-    km, kmin = time_launches(step, s, 20)
+    km = time_launches(step, s, 20)
     out["small"] = {"kernel_ms": km, "gsps": small / (km * 1e-3) / 1e9,
                     "gbs": small * 8 / (km * 1e-3) / 1e9}
     buf.free()
@@ -134,7 +133,7 @@ def bench_batched(args):
     cfg = b.autotune(stream=s, cache_path=os.path.join(tempfile.gettempdir(), "mrg8_batch_tune.txt"))
     for _ in range(2):
         b.run(s)
-    km, kmin = time_launches(lambda: b.run(s), s, max(5, args.steps))
+    km = time_launches(lambda: b.run(s), s, max(5, args.steps))
     buf.free()
     gbs = total * 8 / (km * 1e-3) / 1e9
     log(f"batched: {total / (km * 1e-3) / 1e9:.1f} G samples/s, {gbs:.0f} GB/s, cfg {cfg}")
@@ -156,7 +155,7 @@ def bench_jumps(args):
     s = hip.Stream()
     for _ in range(2):
         b.run(s)
-    km, _ = time_launches(lambda: b.run(s), s, max(5, args.steps))
+    km = time_launches(lambda: b.run(s), s, max(5, args.steps))
     buf.free()
     pc = float(np.mean([bin(int(x)).count("1") for x in steps]))
     log(f"jumps: {km:.3f} ms per launch, {n / (km * 1e-3) / 1e6:.1f} M streams/s")
@@ -248,7 +247,7 @@ def main():
     ap.add_argument("--quick", action="store_true", help="single-stream figures only")
     ap.add_argument("--served-requests", type=int, default=100_000)
     ap.add_argument("--served-load", type=float, default=0.7)
-    ap.add_argument("--step-outputs", type=int, default=1 << 26)
+    ap.add_argument("--step-outputs", type=int, default=0, help="server step size (0 = library default)")
     ap.add_argument("--cpu-log2", type=int, default=26)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()

@@ -111,10 +111,28 @@ int mrg8_advance_state_bits(const uint32_t state[8], const short bits[200], uint
  * (n is 64-bit; the reference takes int and jumps only 32 bits on device). */
 int mrg8_generate(mrg8_gen_t g, void* dptr, uint64_t n, int format, void* stream);
 /* Same, with an explicit launch shape: threads per block (64..1024, multiple
- * of 64) and outputs per wave (multiple of 64; 0 = automatic).  The output is
- * identical for every shape. */
+ * of 64), outputs per wave (rounded up to 64 * outputs per lane) and outputs
+ * per lane (1, 2, or 4 for words; 1 or 2 for doubles); 0 means automatic for
+ * each.  The output is identical for every shape. */
 int mrg8_generate_shape(mrg8_gen_t g, void* dptr, uint64_t n, int format, void* stream,
-                        int block_threads, uint64_t outputs_per_wave);
+                        int block_threads, uint64_t outputs_per_wave, int outputs_per_lane);
+/* Launch shape of a single-stream fill. */
+typedef struct mrg8_fill_shape {
+  int32_t block_threads;      /* threads per block */
+  int32_t outputs_per_lane;   /* outputs each lane makes per iteration */
+  uint64_t outputs_per_wave;  /* consecutive outputs per wave (the jump stride between waves) */
+} mrg8_fill_shape;
+
+/* Autotuner for single-stream fills: times threads-per-block x outputs-per-
+ * lane x outputs-per-wave on `device` for fills of about n outputs in
+ * `format` (a scratch buffer of n outputs is allocated for the trials) and
+ * makes the fastest the default shape of mrg8_generate for that format and
+ * size class (floor(log2 n)) on that device.  `cache_path` (may be NULL) is a
+ * text file of results keyed by device, format and size class; a cached
+ * entry is used without timing.  The counterpart of the reference's manual
+ * TNUM x BS sweep (cuda/tests/test_mrg8_outer.cu:50-53). */
+int mrg8_tune_fill(int device, int format, uint64_t n, const char* cache_path, mrg8_fill_shape* best);
+
 /* mrg8_cuda::mrg8_outer_32(d_ran, n[, TNUM[, BS]]) (cuda/mrg8_cuda.cu:414-436):
  * blocking, [0, 1) doubles (MRG8_F64_01).  BS/TNUM keep the reference's
  * meaning (threads per block / total threads) and pick the launch shape. */

@@ -105,11 +105,12 @@ class mrg8_cuda:
         return np.array(out[:], dtype=np.uint32)
 
     # -- device generation ------------------------------------------------------
-    def generate(self, dptr, n: int, fmt: int = F64, stream=None, block_threads: int = 0, tile: int = 0) -> None:
+    def generate(self, dptr, n: int, fmt: int = F64, stream=None, block_threads: int = 0, tile: int = 0,
+                 per_lane: int = 0) -> None:
         """Stream-ordered fill of n outputs at device address `dptr`; the state advances by n."""
-        if block_threads or tile:
+        if block_threads or tile or per_lane:
             capi.call("mrg8_generate_shape", self._h, ctypes.c_void_p(_ptr(dptr)), ctypes.c_uint64(n), fmt,
-                      _stream(stream), block_threads, ctypes.c_uint64(tile))
+                      _stream(stream), block_threads, ctypes.c_uint64(tile), per_lane)
         else:
             capi.call("mrg8_generate", self._h, ctypes.c_void_p(_ptr(dptr)), ctypes.c_uint64(n), fmt,
                       _stream(stream))
@@ -175,6 +176,15 @@ def advance_state(state, step: int = 0, bits=None) -> np.ndarray:
     else:
         capi.call("mrg8_advance_state", _u32x8(state), ctypes.c_uint64(step), out)
     return np.array(out[:], dtype=np.uint32)
+
+
+def tune_fill(fmt: int, n: int, cache_path: str | None = None, device: int = 0):
+    """Autotunes the single-stream fill shape for (fmt, size class of n) on `device`."""
+    sh = capi.FillShape()
+    capi.call("mrg8_tune_fill", device, fmt, ctypes.c_uint64(n), cache_path.encode() if cache_path else None,
+              ctypes.byref(sh))
+    return {"block_threads": sh.block_threads, "outputs_per_lane": sh.outputs_per_lane,
+            "outputs_per_wave": sh.outputs_per_wave}
 
 
 def jump_matrix() -> np.ndarray:

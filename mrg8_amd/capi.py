@@ -26,6 +26,11 @@ class BatchConfig(ctypes.Structure):
                 ("tile_outputs", ctypes.c_uint64)]
 
 
+class FillShape(ctypes.Structure):
+    _fields_ = [("block_threads", ctypes.c_int32), ("outputs_per_lane", ctypes.c_int32),
+                ("outputs_per_wave", ctypes.c_uint64)]
+
+
 class ServerConfig(ctypes.Structure):
     _fields_ = [("num_streams", ctypes.c_uint32), ("max_inflight", ctypes.c_uint32),
                 ("max_output_bytes", ctypes.c_uint64), ("step_outputs", ctypes.c_uint64),
@@ -73,7 +78,10 @@ SIGNATURES = {
     "mrg8_advance_state": (ctypes.c_int, [u32p, ctypes.c_uint64, u32p]),
     "mrg8_advance_state_bits": (ctypes.c_int, [u32p, ctypes.POINTER(ctypes.c_short), u32p]),
     "mrg8_generate": (ctypes.c_int, [H, vp, ctypes.c_uint64, ctypes.c_int, vp]),
-    "mrg8_generate_shape": (ctypes.c_int, [H, vp, ctypes.c_uint64, ctypes.c_int, vp, ctypes.c_int, ctypes.c_uint64]),
+    "mrg8_generate_shape": (ctypes.c_int, [H, vp, ctypes.c_uint64, ctypes.c_int, vp, ctypes.c_int, ctypes.c_uint64,
+                                           ctypes.c_int]),
+    "mrg8_tune_fill": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.c_char_p,
+                                      ctypes.POINTER(FillShape)]),
     "mrg8_outer_32": (ctypes.c_int, [H, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "mrg8_inner": (ctypes.c_int, [H, vp, ctypes.c_int, ctypes.c_int]),
     "mrg8_state_cpy_DtH": (ctypes.c_int, [H]),

@@ -219,7 +219,7 @@ extern "C" int mrg8_batch_autotune(mrg8_batch_t b, void* stream, const char* cac
   // reference's manual TNUM x BS sweep (cuda/tests/test_mrg8_outer.cu:50-53)
   const int blocks[] = {256, 512, 1024};
   const int waves_per_cu[] = {16, 32};
-  const uint64_t tiles[] = {16384, 65536, 262144};
+  const uint64_t tiles[] = {2048, 4096, 8192, 16384, 65536};
   double best_ms = 1e30;
   mrg8_batch_config best_cfg = b->cfg;
   int rc = MRG8_OK;

@@ -74,7 +74,7 @@ int device_context(int device, DeviceContext** out) {
   if ((e = hipSetDevice(device)) != hipSuccess) return hip_fail(e, "hipSetDevice");
 
   // Coefficient records: block rows (row 0 of A^(k+1)) and jump rows.
-  std::vector<CoefRecord> lane(kLaneRows), jump((size_t)kJumpBits * 8);
+  std::vector<CoefRecord> lane(kLaneRows), jump((size_t)kJumpDigits * 15 * 8);
   const std::vector<uint32_t> rows = block_rows(kLaneRows);
   for (int k = 0; k < kLaneRows; ++k) {
     if (!all_nonzero(&rows[(size_t)k * 8])) {
@@ -83,14 +83,18 @@ int device_context(int device, DeviceContext** out) {
     }
     lane[k] = make_record(&rows[(size_t)k * 8]);
   }
+  // Radix-16 jump table: A^(d 16^k) for digit d = 1..15 at position k, built
+  // from A^(16^k) = A^(2^(4k)) by repeated products.  Rows of the small
+  // powers (A^1..A^7) have structural zeros; the support mask keeps their
+  // exact-zero results exact.
   const JumpTable& jt = host_jump_table();
-  for (int b = 0; b < kJumpBits; ++b)
-    for (int r = 0; r < 8; ++r) {
-      const uint32_t* row = jt.at(b) + r * 8;
-      // Rows are either fully non-zero or (for A, A^2, A^4) a shifted unit
-      // vector; both are handled exactly by the support mask.
-      jump[(size_t)b * 8 + r] = make_record(row);
+  for (int k = 0; k < kJumpDigits; ++k) {
+    std::vector<uint32_t> m(jt.at(4 * k), jt.at(4 * k) + 64), p = m;
+    for (int d = 1; d <= 15; ++d) {
+      if (d > 1) p = mat_mul(p.data(), m.data());
+      for (int r = 0; r < 8; ++r) jump[((size_t)k * 15 + (d - 1)) * 8 + r] = make_record(&p[(size_t)r * 8]);
     }
+  }
 
   auto ctx = std::make_unique<DeviceContext>();
   ctx->device = device;
@@ -152,12 +156,32 @@ void default_shape(const DeviceContext& ctx, uint64_t n, int format, int per_lan
 }
 
 int fill_device(DeviceContext& ctx, const uint32_t state[8], uint64_t offset, void* dptr, uint64_t n, int format,
-                hipStream_t stream, int block_threads, uint64_t tile) {
+                hipStream_t stream, int block_threads, uint64_t tile, int per_lane) {
   if (n == 0) return MRG8_OK;
-  const int pl = pick_per_lane(format, dptr);
+  const int esz = format_size(format);
+  // Outputs before the first 16-byte boundary are written by wave 0 of block 0
+  // (peel); the waves then cover the aligned remainder with wide stores.
+  uint32_t peel = (uint32_t)(((16u - (reinterpret_cast<uintptr_t>(dptr) & 15u)) & 15u) / (uint32_t)esz);
+  if (peel > n) peel = (uint32_t)n;
+  void* main_ptr = static_cast<char*>(dptr) + (size_t)peel * esz;
+  const uint64_t main_n = n - peel;
+  int pl = pick_per_lane(format, main_ptr);
+  if (per_lane) {
+    if (per_lane != 1 && per_lane != 2 && per_lane != 4) return fail(MRG8_ERR_ARG, "outputs_per_lane must be 1, 2 or 4");
+    if (format != MRG8_U32 && per_lane == 4) return fail(MRG8_ERR_ARG, "doubles take at most 2 outputs per lane");
+    pl = per_lane;
+  }
   int def_block = 0;
   uint64_t def_tile = 0;
-  default_shape(ctx, n, format, pl, &def_block, &def_tile);
+  const int cls = main_n ? 63 - __builtin_clzll(main_n) : 0;
+  const TunedShape& tuned = ctx.tuned[format][cls];
+  if (tuned.valid && !per_lane && (tuned.per_lane == 1 || pick_per_lane(format, main_ptr) >= tuned.per_lane)) {
+    pl = tuned.per_lane;
+    def_block = tuned.block_threads;
+    def_tile = tuned.tile;
+  } else {
+    default_shape(ctx, main_n, format, pl, &def_block, &def_tile);
+  }
   if (block_threads <= 0) block_threads = def_block;
   if (block_threads < 64 || block_threads > 1024 || block_threads % 64)
     return fail(MRG8_ERR_ARG, "block_threads %d must be a multiple of 64 in [64, 1024]", block_threads);
@@ -165,9 +189,11 @@ int fill_device(DeviceContext& ctx, const uint32_t state[8], uint64_t offset, vo
   if (tile == 0) tile = def_tile;
   if (tile % quantum) tile = (tile + quantum - 1) / quantum * quantum;
   FillArgs a{};
-  a.out = dptr;
-  a.n = n;
+  a.out = main_ptr;
+  a.n = main_n;
   a.tile = tile;
+  a.peel = peel;
+  a.peel_out = dptr;
   std::memcpy(a.start.state, state, sizeof a.start.state);
   a.start.offset = offset;
   a.lane = ctx.tables.lane;
@@ -342,7 +368,7 @@ extern "C" int mrg8_advance_state_bits(const uint32_t state[8], const short bits
 }
 
 static int generate_impl(mrg8_gen_t g, void* dptr, uint64_t n, int format, void* stream, int block_threads,
-                         uint64_t tile) {
+                         uint64_t tile, int per_lane) {
   if (!g) return fail(MRG8_ERR_ARG, "null generator");
   const int esz = format_size(format);
   if (!esz) return fail(MRG8_ERR_ARG, "unknown format %d", format);
@@ -352,7 +378,7 @@ static int generate_impl(mrg8_gen_t g, void* dptr, uint64_t n, int format, void*
     return fail(MRG8_ERR_ARG, "output pointer must be %d-byte aligned", esz);
   if (n > (UINT64_MAX / 8)) return fail(MRG8_ERR_ARG, "n too large");
   hipStream_t s = static_cast<hipStream_t>(stream);
-  int rc = fill_device(*g->ctx, g->state, 0, dptr, n, format, s, block_threads, tile);
+  int rc = fill_device(*g->ctx, g->state, 0, dptr, n, format, s, block_threads, tile, per_lane);
   if (rc) return rc;
   g->last_stream = s;
   g->issued = true;
@@ -362,13 +388,13 @@ static int generate_impl(mrg8_gen_t g, void* dptr, uint64_t n, int format, void*
 
 extern "C" int mrg8_generate(mrg8_gen_t g, void* dptr, uint64_t n, int format, void* stream) {
   clear_error();
-  return generate_impl(g, dptr, n, format, stream, 0, 0);
+  return generate_impl(g, dptr, n, format, stream, 0, 0, 0);
 }
 
 extern "C" int mrg8_generate_shape(mrg8_gen_t g, void* dptr, uint64_t n, int format, void* stream,
-                                   int block_threads, uint64_t outputs_per_wave) {
+                                   int block_threads, uint64_t outputs_per_wave, int outputs_per_lane) {
   clear_error();
-  return generate_impl(g, dptr, n, format, stream, block_threads, outputs_per_wave);
+  return generate_impl(g, dptr, n, format, stream, block_threads, outputs_per_wave, outputs_per_lane);
 }
 
 extern "C" int mrg8_synchronize(mrg8_gen_t g) {
@@ -398,7 +424,7 @@ static int drop_in_fill(mrg8_gen_t g, double* d, int n, int bs, int tnum) {
   uint64_t tile = ((uint64_t)n + waves - 1) / waves;
   tile = std::max<uint64_t>(quantum, (tile + quantum - 1) / quantum * quantum);
   int block = bs < 64 ? 64 : (bs > 1024 ? 1024 : bs / 64 * 64);
-  int rc = generate_impl(g, d, (uint64_t)n, MRG8_F64_01, nullptr, block, tile);
+  int rc = generate_impl(g, d, (uint64_t)n, MRG8_F64_01, nullptr, block, tile, 0);
   if (rc) return rc;
   return mrg8_synchronize(g);
 }
@@ -445,7 +471,7 @@ static int host_fill(mrg8_gen_t g, void* host, uint64_t n, int format) {
   int rc = MRG8_OK;
   for (uint64_t done = 0; done < n && rc == MRG8_OK;) {
     const uint64_t m = std::min<uint64_t>(per, n - done);
-    rc = fill_device(*g->ctx, g->state, 0, g->staging, m, format, nullptr, 0, 0);
+    rc = fill_device(*g->ctx, g->state, 0, g->staging, m, format, nullptr, 0, 0, 0);
     if (rc) break;
     jump64(host_jump_table(), m, g->state);
     e = hipMemcpy(static_cast<char*>(host) + done * esz, g->staging, m * esz, hipMemcpyDeviceToHost);
@@ -513,5 +539,118 @@ extern "C" int mrg8_write_jump_matrix(const char* path) {
     std::fputc('\n', f);
   }
   if (std::fclose(f) != 0) return fail(MRG8_ERR_IO, "cannot write %s", path);
+  return MRG8_OK;
+}
+
+// ---- fill autotuning -------------------------------------------------------------
+
+namespace {
+
+std::string fill_key(const DeviceContext& ctx, int format, int cls) {
+  hipDeviceProp_t prop;
+  (void)hipGetDeviceProperties(&prop, ctx.device);
+  char buf[256];
+  std::snprintf(buf, sizeof buf, "%s:%d:fill:fmt%d:n2^%d", prop.gcnArchName, prop.multiProcessorCount, format, cls);
+  std::string k = buf;
+  for (char& c : k)
+    if (c == ' ') c = '_';
+  return k;
+}
+
+}  // namespace
+
+extern "C" int mrg8_tune_fill(int device, int format, uint64_t n, const char* cache_path, mrg8_fill_shape* best) {
+  clear_error();
+  const int esz = format_size(format);
+  if (!esz) return fail(MRG8_ERR_ARG, "unknown format %d", format);
+  if (n < 4096) return fail(MRG8_ERR_ARG, "tune with at least 4096 outputs");
+  DeviceContext* ctx = nullptr;
+  int rc = device_context(device, &ctx);
+  if (rc) return rc;
+  const int cls = 63 - __builtin_clzll(n);
+  const std::string key = fill_key(*ctx, format, cls);
+  TunedShape found;
+  if (cache_path) {
+    std::FILE* f = std::fopen(cache_path, "r");
+    if (f) {
+      char k[256];
+      int bt = 0, pl = 0;
+      unsigned long long tile = 0;
+      double ms = 0;
+      while (std::fscanf(f, "%255s %d %d %llu %lf", k, &bt, &pl, &tile, &ms) == 5)
+        if (key == k && bt >= 64 && bt <= 1024 && bt % 64 == 0 && (pl == 1 || pl == 2 || pl == 4) && tile) {
+          found.valid = true;
+          found.block_threads = bt;
+          found.per_lane = pl;
+          found.tile = tile;
+        }
+      std::fclose(f);
+    }
+  }
+  if (!found.valid) {
+    int prev = 0;
+    (void)hipGetDevice(&prev);
+    (void)hipSetDevice(device);
+    void* buf = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipMalloc(&buf, n * (uint64_t)esz);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e != hipSuccess) {
+      if (buf) (void)hipFree(buf);
+      (void)hipSetDevice(prev);
+      return hip_fail(e, "tuning allocation");
+    }
+    uint32_t st[8];
+    seed_state(13579u, st);
+    double best_ms = 1e30;
+    const int blocks[] = {256, 512, 1024};
+    const uint64_t tiles[] = {2048, 4096, 8192, 16384};
+    const int pls[] = {1, 2, 4};
+    for (int pl : pls) {
+      if (pl == 4 && format != MRG8_U32) continue;
+      for (int bt : blocks)
+        for (uint64_t tile : tiles) {
+          if (tile < 64ull * pl * 8) continue;
+          rc = fill_device(*ctx, st, 0, buf, n, format, nullptr, bt, tile, pl);  // warm-up
+          if (rc) break;
+          (void)hipEventRecord(e0, nullptr);
+          const int reps = 4;
+          for (int r = 0; r < reps && rc == MRG8_OK; ++r) rc = fill_device(*ctx, st, 0, buf, n, format, nullptr, bt, tile, pl);
+          (void)hipEventRecord(e1, nullptr);
+          (void)hipEventSynchronize(e1);
+          float ms = 0;
+          (void)hipEventElapsedTime(&ms, e0, e1);
+          ms /= reps;
+          if (rc == MRG8_OK && ms < best_ms) {
+            best_ms = ms;
+            found.valid = true;
+            found.block_threads = bt;
+            found.per_lane = pl;
+            found.tile = tile;
+          }
+        }
+      if (rc) break;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(buf);
+    (void)hipSetDevice(prev);
+    if (rc) return rc;
+    if (cache_path && found.valid) {
+      std::FILE* f = std::fopen(cache_path, "a");
+      if (f) {
+        std::fprintf(f, "%s %d %d %llu %.6f\n", key.c_str(), found.block_threads, found.per_lane,
+                     (unsigned long long)found.tile, best_ms);
+        std::fclose(f);
+      }
+    }
+  }
+  ctx->tuned[format][cls] = found;
+  if (best) {
+    best->block_threads = found.block_threads;
+    best->outputs_per_lane = found.per_lane;
+    best->outputs_per_wave = found.tile;
+  }
   return MRG8_OK;
 }

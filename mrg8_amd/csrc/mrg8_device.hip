@@ -66,29 +66,45 @@ __device__ __forceinline__ void centre(const uint32_t s[8], int32_t e[8]) {
   for (int j = 0; j < 8; ++j) e[j] = (int32_t)(s[j] - kCenter);
 }
 
-// Advances the wave-uniform state by `off` positions with the reference's
-// exact semantics (words of A^(2^b) s, a residue of 0 written as p unless the
-// exact sum is 0).
-__device__ void jump_uniform(uint32_t s[8], uint64_t off, const CoefRecord* __restrict__ jrec) {
+// Advances eight states at once: lane group g = lane / 8 holds its own state
+// (replicated in the group's lanes, s[] per lane) and its own 64-bit offset;
+// for the group's lowest non-zero hexadecimal digit d at position k, lane r
+// of the group computes row r of A^(d 16^k), and the group's eight new words
+// are gathered back with ds_bpermute.  Groups with no digits left idle until
+// every group is done; a 64-bit jump takes at most 16 steps.
+__device__ __forceinline__ const CoefRecord* digit_row(const CoefRecord* __restrict__ jrec, uint64_t off, int row) {
+  const int k = __builtin_ctzll(off) >> 2;
+  const int d = (int)((off >> (4 * k)) & 15u);
+  return jrec + ((k * 15 + (d - 1)) * 8 + row);
+}
+
+__device__ __forceinline__ uint64_t drop_digit(uint64_t off) {
+  return off & ~(15ull << (__builtin_ctzll(off) & ~3));
+}
+
+__device__ void jump_groups(uint32_t s[8], uint64_t off, const CoefRecord* __restrict__ jrec) {
   const int lane = threadIdx.x & 63;
   const int grp = lane >> 3, row = lane & 7;
-  while (off) {
-    uint64_t x = off;
-    for (int i = 0; i < grp; ++i) x &= x - 1;  // this group's set bit
-    const int nb = min(__popcll(off), 8);
-    Coef k = {};
-    if (x) k = load_coef(jrec + (__builtin_ctzll(x) * 8 + row));
-    for (int t = 0; t < nb; ++t) {
-      int32_t e[8];
-      uint32_t nz = 0;
-      centre(s, e);
+  Coef k = {};
+  if (off) k = load_coef(digit_row(jrec, off, row));
+  while (__ballot(off != 0)) {
+    const bool active = off != 0;
+    const uint64_t rest = active ? drop_digit(off) : 0;
+    Coef next = {};
+    if (rest) next = load_coef(digit_row(jrec, rest, row));  // one step ahead
+    int32_t e[8];
+    uint32_t nz = 0;
+    centre(s, e);
 #pragma unroll
-      for (int j = 0; j < 8; ++j) nz |= (s[j] != 0u ? 1u : 0u) << j;
-      const uint32_t y = (k.support & nz) ? dot_residue(k, e) + 1u : 0u;
+    for (int j = 0; j < 8; ++j) nz |= (s[j] != 0u ? 1u : 0u) << j;
+    const uint32_t y = (k.support & nz) ? dot_residue(k, e) + 1u : 0u;
 #pragma unroll
-      for (int j = 0; j < 8; ++j) s[j] = __builtin_amdgcn_readlane(y, 8 * t + j);
+    for (int j = 0; j < 8; ++j) {
+      const uint32_t w = (uint32_t)__builtin_amdgcn_ds_bpermute((grp * 8 + j) * 4, (int)y);
+      s[j] = active ? w : s[j];
     }
-    for (int t = 0; t < nb; ++t) off &= off - 1;
+    off = rest;
+    k = next;
   }
 }
 
@@ -178,85 +194,117 @@ __device__ __forceinline__ void generate_run(typename Out<FMT>::T* __restrict__ 
   }
 }
 
+// Writes the first k (< 64) outputs of a run one per lane and advances the
+// wave-uniform state past them: used to reach 16-byte alignment before the
+// wide-store loop.
+template <int FMT>
+__device__ __forceinline__ void peel_outputs(typename Out<FMT>::T* __restrict__ dst, uint32_t k, uint32_t s[8],
+                                             const CoefRecord* __restrict__ lane_rec) {
+  const uint32_t lane = threadIdx.x & 63;
+  uint32_t any = 0;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) any |= s[j];
+  if (any == 0u) {  // the all-zero state stays zero
+    if (lane < k) __builtin_nontemporal_store(convert_zero<FMT>(), dst + lane);
+    return;
+  }
+  const Coef c = load_coef(lane_rec + lane);  // row 0 of A^(lane+1)
+  int32_t e[8];
+  centre(s, e);
+  const uint32_t r = dot_residue(c, e);
+  if (lane < k) __builtin_nontemporal_store(convert<FMT>(r), dst + lane);
+  // new state: the k new words (newest first) followed by the 8-k oldest kept
+  uint32_t ns[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const uint32_t src = (uint32_t)j < k ? k - 1 - (uint32_t)j : 0u;
+    const uint32_t fresh = __builtin_amdgcn_readlane(r, src) + 1u;
+    uint32_t kept = s[0];
+#pragma unroll
+    for (int q = 1; q < 8; ++q) kept = (uint32_t)(j - q) == k ? s[q] : kept;  // s[j - k]
+    kept = (uint32_t)j == k ? s[0] : kept;
+    ns[j] = (uint32_t)j < k ? fresh : kept;
+  }
+#pragma unroll
+  for (int j = 0; j < 8; ++j) s[j] = ns[j];
+}
+
 template <int FMT, int PL>
 __global__ __launch_bounds__(1024) void k_fill(FillArgs a) {
-  // wave index, made explicitly uniform so the loop state stays scalar
-  const uint32_t wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const uint64_t wave = (uint64_t)blockIdx.x * (blockDim.x >> 6) + wib;
-  const uint64_t first = wave * a.tile;
+  // Start states of the block's waves: wave j (j < ceil(nw / 8)) jumps the
+  // states of waves 8j..8j+7 together (one 8-lane group each) and leaves
+  // them in LDS; every wave then reads its own.
+  __shared__ uint32_t start_state[16][8];
+  const int lane = threadIdx.x & 63;
+  const uint32_t wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform wave index
+  const uint32_t nw = blockDim.x >> 6;
+  const uint64_t wave0 = (uint64_t)blockIdx.x * nw;
+  if (wib * 8 < nw) {
+    const uint32_t target = wib * 8 + (lane >> 3);
+    uint32_t s[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) s[j] = a.start.state[j];
+    jump_groups(s, target < nw ? a.start.offset + a.peel + (wave0 + target) * a.tile : 0, a.jump);
+    if (target < nw && (lane & 7) == 0) {
+#pragma unroll
+      for (int j = 0; j < 8; ++j) start_state[target][j] = s[j];
+    }
+  }
+  __syncthreads();
+  if (a.peel && blockIdx.x == 0 && wib == 0) {  // the outputs before the first 16-byte boundary
+    uint32_t s[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) s[j] = a.start.state[j];
+    jump_groups(s, lane < 8 ? a.start.offset : 0, a.jump);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) s[j] = __builtin_amdgcn_readfirstlane(s[j]);
+    peel_outputs<FMT>(static_cast<typename Out<FMT>::T*>(a.peel_out), a.peel, s, a.lane);
+  }
+  const uint64_t first = (wave0 + wib) * a.tile;
   if (first >= a.n) return;
   const uint64_t cnt = min(a.tile, a.n - first);
   uint32_t s[8];
 #pragma unroll
-  for (int j = 0; j < 8; ++j) s[j] = a.start.state[j];
-  const uint64_t off = a.start.offset + first;
-  if (off) jump_uniform(s, off, a.jump);
+  for (int j = 0; j < 8; ++j) s[j] = __builtin_amdgcn_readfirstlane(start_state[wib][j]);
   generate_run<FMT, PL>(static_cast<typename Out<FMT>::T*>(a.out) + first, cnt, s, a.lane);
 }
 
-// One tile of a batch segment: picks the store width from the destination's
-// alignment (segments may start at any element).
+// Outputs needed before `p` is 16-byte aligned (p itself element-aligned).
+template <typename T>
+__device__ __forceinline__ uint32_t peel_count(const T* p) {
+  return (uint32_t)(((16u - (reinterpret_cast<uintptr_t>(p) & 15u)) & 15u) / sizeof(T));
+}
+
+// One tile of a batch segment: peels up to one double (three words) to reach
+// 16-byte alignment, then runs the wide-store loop.
 template <int FMT>
 __device__ __forceinline__ void batch_tile(const BatchSegment& sg, uint64_t first, uint64_t cnt, uint32_t s[8],
                                            const CoefRecord* __restrict__ lane_rec) {
   typedef typename Out<FMT>::T T;
   T* dst = static_cast<T*>(sg.out) + first;
-  const uintptr_t addr = reinterpret_cast<uintptr_t>(dst);
-  if constexpr (FMT == 2) {
-    if ((addr & 15u) == 0) return generate_run<FMT, 4>(dst, cnt, s, lane_rec);
-    if ((addr & 7u) == 0) return generate_run<FMT, 2>(dst, cnt, s, lane_rec);
-    return generate_run<FMT, 1>(dst, cnt, s, lane_rec);
-  } else {
-    if ((addr & 15u) == 0) return generate_run<FMT, 2>(dst, cnt, s, lane_rec);
-    return generate_run<FMT, 1>(dst, cnt, s, lane_rec);
+  uint32_t peel = peel_count(dst);
+  if (peel > cnt) peel = (uint32_t)cnt;
+  if (peel) {
+    peel_outputs<FMT>(dst, peel, s, lane_rec);
+    dst += peel;
+    cnt -= peel;
   }
+  if (cnt) generate_run<FMT, 2>(dst, cnt, s, lane_rec);
 }
 
-// Advances eight states at once: lane group g = lane / 8 holds its own state
-// (replicated in the group's lanes, s[] per lane) and its own 64-bit offset;
-// lane r of a group computes row r of A^(2^b) for the group's lowest set bit
-// b, and the group's eight new words are gathered back with ds_bpermute.
-// Groups with no bits left idle until every group is done.
-__device__ void jump_groups(uint32_t s[8], uint64_t off, const CoefRecord* __restrict__ jrec) {
-  const int lane = threadIdx.x & 63;
-  const int grp = lane >> 3, row = lane & 7;
-  Coef k = {};
-  if (off) k = load_coef(jrec + (__builtin_ctzll(off) * 8 + row));
-  while (__ballot(off != 0)) {
-    const bool active = off != 0;
-    const uint64_t rest = active ? off & (off - 1) : 0;
-    Coef next = {};
-    if (rest) next = load_coef(jrec + (__builtin_ctzll(rest) * 8 + row));  // one step ahead
-    int32_t e[8];
-    uint32_t nz = 0;
-    centre(s, e);
-#pragma unroll
-    for (int j = 0; j < 8; ++j) nz |= (s[j] != 0u ? 1u : 0u) << j;
-    const uint32_t y = (k.support & nz) ? dot_residue(k, e) + 1u : 0u;
-#pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      const uint32_t w = (uint32_t)__builtin_amdgcn_ds_bpermute((grp * 8 + j) * 4, (int)y);
-      s[j] = active ? w : s[j];
-    }
-    off = rest;
-    k = next;
-  }
-}
-
-// Persistent batched kernel.  A wave takes `grab` consecutive tiles per
-// counter increment, jumps their states together (jump_groups), then
-// generates the tiles one after another.  Every wave exits once the counter
-// passes ntiles.
+// Persistent batched kernel.  Tiles are taken in groups of `grab`
+// consecutive tiles; wave w of W handles groups w, w + W, w + 2W, ... (tiles
+// are equal-sized, so a static round robin balances without any atomic).
+// The states of a group's tiles are jumped together (jump_groups), then the
+// tiles are generated one after another.
 __global__ __launch_bounds__(1024) void k_batch(BatchArgs a) {
   const int lane = threadIdx.x & 63;
   const int grp = lane >> 3;
   const uint64_t ntiles = a.hdr->ntiles;
   const uint32_t grab = a.hdr->grab;
-  for (;;) {
-    uint32_t t0 = 0;
-    if (lane == 0) t0 = atomicAdd(a.cursor, grab);
-    const uint64_t base = (uint64_t)__builtin_amdgcn_readfirstlane(t0);
-    if (base >= ntiles) return;
+  const uint64_t nwaves = (uint64_t)gridDim.x * (blockDim.x >> 6);
+  const uint64_t wave = (uint64_t)blockIdx.x * (blockDim.x >> 6) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  for (uint64_t base = wave * grab; base < ntiles; base += nwaves * grab) {
     const uint32_t ngrp = (uint32_t)min<uint64_t>(grab, ntiles - base);
     // per group: its tile, segment, starting state and offset
     const bool mine = (uint32_t)grp < ngrp;
@@ -296,7 +344,8 @@ template <int FMT, int PL>
 hipError_t fill_one(const FillArgs& a, int block_threads, hipStream_t stream) {
   const uint64_t waves = (a.n + a.tile - 1) / a.tile;
   const uint64_t wpb = (uint64_t)block_threads / 64;
-  const uint64_t blocks = (waves + wpb - 1) / wpb;
+  uint64_t blocks = (waves + wpb - 1) / wpb;
+  if (blocks == 0 && a.peel) blocks = 1;  // only the peeled outputs
   if (blocks == 0) return hipSuccess;
   if (blocks > 0x7fffffffull) return hipErrorInvalidConfiguration;
   hipLaunchKernelGGL((k_fill<FMT, PL>), dim3((uint32_t)blocks), dim3(block_threads), 0, stream, a);

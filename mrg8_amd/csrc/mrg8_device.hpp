@@ -10,12 +10,12 @@
 namespace mrg8 {
 
 constexpr int kLaneRows = 256;   // block rows kept on the device: up to 4 outputs per lane
-constexpr int kJumpBits = 64;    // device jumps take 64-bit offsets
+constexpr int kJumpDigits = 16;  // device jumps take 64-bit offsets as 16 hexadecimal digits
 
 // Per-device constant tables.
 struct DeviceTables {
   CoefRecord* lane = nullptr;  // kLaneRows records: row 0 of A^(k+1), support 0xff
-  CoefRecord* jump = nullptr;  // kJumpBits * 8 records: row r of A^(2^b)
+  CoefRecord* jump = nullptr;  // kJumpDigits * 15 * 8 records: row r of A^(d 16^k) at ((k*15)+(d-1))*8+r
 };
 
 // Wave-uniform starting point of a launch.
@@ -24,13 +24,16 @@ struct StreamStart {
   uint64_t offset;    // jump applied before output 0
 };
 
-// One fill launch over a single stream: output i of the launch is stream
-// position start.offset + i.  Each wave produces `tile` consecutive outputs.
+// One fill launch over a single stream: peel_out[i] (i < peel) is stream
+// position start.offset + i and out[i] is position start.offset + peel + i.
+// Each wave produces `tile` consecutive outputs of `out`.
 struct FillArgs {
-  void* out;
+  void* out;                // 16-byte aligned unless outputs-per-lane is 1
   uint64_t n;
   uint64_t tile;            // multiple of 64 * outputs-per-lane
-  StreamStart start;
+  uint32_t peel;            // outputs written to peel_out[0..peel) before out[0] (at most 3)
+  void* peel_out;
+  StreamStart start;        // the state, and the position (offset) of the first output
   const CoefRecord* lane;
   const CoefRecord* jump;
 };

@@ -11,10 +11,18 @@
 
 namespace mrg8 {
 
+struct TunedShape {
+  bool valid = false;
+  int block_threads = 0;
+  int per_lane = 0;
+  uint64_t tile = 0;
+};
+
 struct DeviceContext {
   int device = 0;
   int num_cu = 0;
   DeviceTables tables;
+  TunedShape tuned[3][64];  // [format][floor(log2 n)], set by mrg8_tune_fill
 };
 
 // Records `fmt` as this thread's last error message and returns `code`.
@@ -29,9 +37,10 @@ int format_size(int format);
 int pick_per_lane(int format, const void* p);
 
 // Fills dptr[0..n) with stream positions offset..offset+n-1 of the stream
-// whose state is `state`; stream-ordered.  block_threads/tile 0 = default.
+// whose state is `state`; stream-ordered.  block_threads/tile/per_lane 0 =
+// default.
 int fill_device(DeviceContext& ctx, const uint32_t state[8], uint64_t offset, void* dptr, uint64_t n, int format,
-                hipStream_t stream, int block_threads, uint64_t tile);
+                hipStream_t stream, int block_threads, uint64_t tile, int per_lane);
 
 // Host wall clock in seconds (steady).
 double now_seconds();

@@ -80,6 +80,18 @@ struct JumpTable {
   const uint32_t* at(int j) const { return m.data() + (size_t)j * 64; }
 };
 
+// Row-major 8x8 product with the reference's dot-product semantics.
+inline std::vector<uint32_t> mat_mul(const uint32_t* x, const uint32_t* y) {
+  std::vector<uint32_t> out(64);
+  for (int r = 0; r < 8; ++r)
+    for (int c = 0; c < 8; ++c) {
+      uint32_t col[8];
+      for (int q = 0; q < 8; ++q) col[q] = y[q * 8 + c];
+      out[(size_t)r * 8 + c] = dot8(x + r * 8, col);
+    }
+  return out;
+}
+
 // state <- A^(2^b) state with the reference's per-row semantics.
 inline void apply_pow2(const JumpTable& t, int b, uint32_t st[8]) {
   const uint32_t* m = t.at(b);

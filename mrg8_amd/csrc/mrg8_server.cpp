@@ -241,10 +241,12 @@ extern "C" int mrg8_server_create(int device, const mrg8_server_config* cfg, uin
   s->ctx = ctx;
   s->cfg = *cfg;
   if (!s->cfg.max_inflight) s->cfg.max_inflight = 4096;
-  if (!s->cfg.tile_outputs) s->cfg.tile_outputs = 65536;
-  if (!s->cfg.step_outputs) s->cfg.step_outputs = 1ull << 28;
-  if (!s->cfg.block_threads) s->cfg.block_threads = 256;
-  if (!s->cfg.blocks_per_cu) s->cfg.blocks_per_cu = 8;
+  // A step should give every wave of the persistent grid several tiles:
+  // 4096-output tiles, 2^27-output steps (32768 tiles for 8192 waves).
+  if (!s->cfg.tile_outputs) s->cfg.tile_outputs = 4096;
+  if (!s->cfg.step_outputs) s->cfg.step_outputs = 1ull << 27;
+  if (!s->cfg.block_threads) s->cfg.block_threads = 1024;
+  if (!s->cfg.blocks_per_cu) s->cfg.blocks_per_cu = 2;
   if (s->cfg.tile_outputs % 256 || s->cfg.block_threads % 64 || s->cfg.block_threads > 1024 ||
       s->cfg.blocks_per_cu * (s->cfg.block_threads / 64) > 32)
     return fail(MRG8_ERR_ARG, "invalid launch configuration");

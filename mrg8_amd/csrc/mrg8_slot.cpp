@@ -88,9 +88,9 @@ uint64_t BatchSlot::finalise(uint32_t count, uint64_t tile, uint64_t grid_waves)
     t += n;
   }
   h_tiles_[count] = t;
-  // Take several tiles per counter increment only when there are enough
-  // tiles to keep every wave busy (their jumps then share one pass).
-  uint64_t grab = grid_waves ? t / (grid_waves * 4) : 1;
+  // Group tiles (their jumps then share one pass) only as far as every wave
+  // of the grid still gets a group.
+  uint64_t grab = grid_waves ? t / grid_waves : 1;
   grab = grab < 1 ? 1 : (grab > 8 ? 8 : grab);
   h_hdr_->nsegs = count;
   h_hdr_->grab = (uint32_t)grab;

@@ -110,7 +110,7 @@ def test_batch_autotune_keeps_outputs(gpu, oracle_mod, tmp_path):
     say("autotune")
     cache = str(tmp_path / "tune.txt")
     cfg = b.autotune(cache_path=cache)
-    assert cfg["tile_outputs"] in (16384, 65536, 262144)
+    assert cfg["tile_outputs"] in (2048, 4096, 8192, 16384, 65536)
     assert os.path.exists(cache)
     cfg2 = b.autotune(cache_path=cache)  # second call is served from the cache
     assert cfg2 == cfg

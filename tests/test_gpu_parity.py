@@ -64,6 +64,29 @@ def test_unaligned_outputs(gpu, oracle_mod, pad, fmt):
     assert np.array_equal(got.view(np.uint8), want.view(np.uint8))
 
 
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 129])
+@pytest.mark.parametrize("pad", [1, 2, 3])
+def test_tiny_unaligned_outputs(gpu, oracle_mod, n, pad):
+    """Fills shorter than the alignment peel (only wave 0's peel writes)."""
+    for fmt in (0, 1, 2):
+        say("tiny-unaligned", n, pad, fmt)
+        g = gpu.mrg8_cuda(SEED)
+        got = run_device(gpu, g, n, fmt, pad_elems=pad)
+        want, st = oracle_mod.generate(oracle_mod.seed_state(SEED), n, fmt)
+        assert np.array_equal(got.view(np.uint8), want.view(np.uint8))
+        assert np.array_equal(g.get_state(), st)
+
+
+@pytest.mark.parametrize("fmt,per_lane", [(0, 1), (0, 2), (2, 1), (2, 2), (2, 4)])
+def test_outputs_per_lane(gpu, oracle_mod, fmt, per_lane):
+    n = 777_777
+    say("per-lane", fmt, per_lane)
+    g = gpu.mrg8_cuda(SEED)
+    got = run_device(gpu, g, n, fmt, pad_elems=1, block_threads=512, tile=8192, per_lane=per_lane)
+    want, _ = oracle_mod.generate(oracle_mod.seed_state(SEED), n, fmt)
+    assert np.array_equal(got.view(np.uint8), want.view(np.uint8))
+
+
 @pytest.mark.parametrize("block,tile", [(64, 128), (256, 1024), (512, 4096), (1024, 65536), (256, 1 << 20)])
 def test_launch_shapes_identical(gpu, oracle_mod, block, tile):
     n = 1_000_003
@@ -194,3 +217,20 @@ def test_c2_sampled_and_wsum(gpu, oracle_mod):
         assert oracle_mod.fnv1a(got) == h
         assert np.array_equal(g.get_state(), st)
         del got
+
+
+def test_fill_autotune_keeps_outputs(gpu, oracle_mod, tmp_path):
+    """The tuned shape becomes the default for that size class; outputs stay bit-exact."""
+    cache = str(tmp_path / "fill_tune.txt")
+    for fmt in (0, 2):
+        say("tune-fill", fmt)
+        shape = gpu.tune_fill(fmt, 1 << 22, cache_path=cache)
+        assert shape["block_threads"] in (256, 512, 1024) and shape["outputs_per_lane"] in (1, 2, 4)
+        again = gpu.tune_fill(fmt, 1 << 22, cache_path=cache)  # served from the cache file
+        assert again == shape
+        g = gpu.mrg8_cuda(SEED)
+        n = (1 << 22) + 12345
+        got = run_device(gpu, g, n, fmt)
+        want, _ = oracle_mod.generate(oracle_mod.seed_state(SEED), n, fmt)
+        assert np.array_equal(got.view(np.uint8), want.view(np.uint8))
+    assert len(open(cache).read().splitlines()) == 2
