@@ -178,13 +178,20 @@ def bench_served(args, capacity_gsps):
         srv.set_stream(i, states[i], 0)
     # warm-up trace (separate streams' positions advance; states stay resident)
     srv.replay(arrivals[:200] / 10, ids[:200], sizes[:200], fmt=mrg8_amd.F64)
+    c0 = srv.counters()
     stats = srv.replay(arrivals, ids, sizes, fmt=mrg8_amd.F64)
+    c1 = srv.counters()
     srv.close()
+    steps = max(1, c1["steps"] - c0["steps"])
     out = {"requests": n_req, "streams": len(states), "offered_load": args.served_load,
            "arrival_rate_per_s": rate, "outputs": int(stats["outputs"]), "wall_s": stats["wall_s"],
            "gsps": stats["outputs"] / stats["wall_s"] / 1e9, "p50_ms": stats["p50_s"] * 1e3,
            "p90_ms": stats["p90_s"] * 1e3, "p99_ms": stats["p99_s"] * 1e3, "max_ms": stats["max_s"] * 1e3,
-           "admission_cap_bytes": 64 << 30, "step_outputs": args.step_outputs}
+           "admission_cap_bytes": 64 << 30,
+           "steps": steps, "mean_step_outputs": (c1["step_outputs"] - c0["step_outputs"]) / steps,
+           "mean_step_segments": (c1["step_segments"] - c0["step_segments"]) / steps,
+           "host_us_per_step": (c1["host_launch_s"] - c0["host_launch_s"]) / steps * 1e6,
+           "max_admission_wait_ms": c1["max_queue_s"] * 1e3}
     log(f"served: {out['gsps']:.1f} G samples/s, p50 {out['p50_ms']:.3f} ms, p99 {out['p99_ms']:.3f} ms")
     return out
 

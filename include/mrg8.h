@@ -251,6 +251,17 @@ int mrg8_server_release(mrg8_server_t s, uint64_t request_id);
 int mrg8_server_stats(mrg8_server_t s, uint64_t* queued, uint64_t* inflight, uint64_t* completed,
                       uint64_t* bytes_inflight);
 
+/* Scheduler counters since creation: launched steps, outputs and segments
+ * they carried, and host seconds spent building and launching them. */
+typedef struct mrg8_server_counters {
+  uint64_t steps;
+  uint64_t step_outputs;
+  uint64_t step_segments;
+  double host_launch_s;
+  double max_queue_s;  /* longest time a request waited for admission */
+} mrg8_server_counters;
+int mrg8_server_counters_get(mrg8_server_t s, mrg8_server_counters* c);
+
 /* Trace replay: a load generator for the served benchmark.  Requests are
  * submitted when the wall clock passes their arrival time (seconds from the
  * start of the replay, non-decreasing), each completion is released at once,

@@ -308,6 +308,11 @@ class Server:
         capi.call("mrg8_server_stats", self._h, *[ctypes.byref(x) for x in v])
         return dict(zip(("queued", "inflight", "completed", "bytes_inflight"), [x.value for x in v]))
 
+    def counters(self):
+        c = capi.ServerCounters()
+        capi.call("mrg8_server_counters_get", self._h, ctypes.byref(c))
+        return {k: getattr(c, k) for k, _ in capi.ServerCounters._fields_}
+
     def replay(self, arrivals, streams, counts, fmt: int = F64, want_latencies: bool = False):
         """Replays a trace (arrival seconds, stream ids, sizes); returns stats (and latencies)."""
         n = len(arrivals)

@@ -44,6 +44,11 @@ class Completion(ctypes.Structure):
                 ("submit_time_s", ctypes.c_double), ("complete_time_s", ctypes.c_double)]
 
 
+class ServerCounters(ctypes.Structure):
+    _fields_ = [("steps", ctypes.c_uint64), ("step_outputs", ctypes.c_uint64), ("step_segments", ctypes.c_uint64),
+                ("host_launch_s", ctypes.c_double), ("max_queue_s", ctypes.c_double)]
+
+
 class TraceEntry(ctypes.Structure):
     _fields_ = [("arrival_s", ctypes.c_double), ("stream_id", ctypes.c_uint32), ("format", ctypes.c_int32),
                 ("count", ctypes.c_uint64)]
@@ -112,6 +117,7 @@ SIGNATURES = {
     "mrg8_server_release": (ctypes.c_int, [H, ctypes.c_uint64]),
     "mrg8_server_stats": (ctypes.c_int, [H, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64),
                                          ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)]),
+    "mrg8_server_counters_get": (ctypes.c_int, [H, ctypes.POINTER(ServerCounters)]),
     "mrg8_server_replay": (ctypes.c_int, [H, ctypes.POINTER(TraceEntry), ctypes.c_uint64,
                                           ctypes.POINTER(ctypes.c_double), ctypes.POINTER(TraceStats)]),
 }

@@ -131,6 +131,7 @@ struct mrg8_server {
   uint64_t next_id = 1;
   uint64_t n_completed = 0;
   uint32_t n_admitted = 0;  // active + issued (not yet complete)
+  mrg8_server_counters counters{};
 };
 
 namespace {
@@ -165,6 +166,7 @@ void admit(mrg8_server* s) {
     r->arena_off = off;
     r->out = s->d_arena + off;
     r->st = RState::Active;
+    s->counters.max_queue_s = std::max(s->counters.max_queue_s, wall_now() - r->t_submit);
     s->queued.pop_front();
     s->active.push_back(r);
     ++s->n_admitted;
@@ -180,6 +182,7 @@ int launch_steps(mrg8_server* s) {
         break;
       }
     if (!step) return MRG8_OK;
+    const double t_build = wall_now();
     int rc = step->slot.wait_staging();
     if (rc) return rc;
     BatchSegment* segs = step->slot.segs();
@@ -215,6 +218,10 @@ int launch_steps(mrg8_server* s) {
     if ((rc = step->slot.run(s->launch, s->stream)) != MRG8_OK) return rc;
     step->busy = true;
     s->inflight.push_back(step);
+    s->counters.steps += 1;
+    s->counters.step_outputs += s->cfg.step_outputs - budget;
+    s->counters.step_segments += n;
+    s->counters.host_launch_s += wall_now() - t_build;
   }
   return MRG8_OK;
 }
@@ -410,6 +417,13 @@ extern "C" int mrg8_server_stats(mrg8_server_t s, uint64_t* queued, uint64_t* in
   if (inflight) *inflight = s->n_admitted;
   if (completed) *completed = s->n_completed;
   if (bytes_inflight) *bytes_inflight = s->arena.used();
+  return MRG8_OK;
+}
+
+extern "C" int mrg8_server_counters_get(mrg8_server_t s, mrg8_server_counters* c) {
+  clear_error();
+  if (!s || !c) return fail(MRG8_ERR_ARG, "null argument");
+  *c = s->counters;
   return MRG8_OK;
 }
 
