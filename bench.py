@@ -114,6 +114,13 @@ def bench_single(args):
     km = time_launches(step, s, 20)
     out["small"] = {"kernel_ms": km, "gsps": small / (km * 1e-3) / 1e9,
                     "gbs": small * 8 / (km * 1e-3) / 1e9}
+    # measured write ceiling: the runtime's own fill kernel over the same 2 GiB
+    # (the counterpart of the reference's stream benchmark, cuda/stream.cu)
+    fill = lambda: hip.memset_async(buf.ptr, 0, N_SINGLE * 8, s)  # noqa: E731
+    fill()
+    km = time_launches(fill, s, 10)
+    out["write_ceiling"] = {"kernel": "hipMemsetAsync 2 GiB", "ms": km, "gbs": N_SINGLE * 8 / (km * 1e-3) / 1e9}
+    log(f"write ceiling (memset): {out['write_ceiling']['gbs']:.0f} GB/s")
     buf.free()
     s.destroy()
     return out
@@ -257,6 +264,7 @@ def main():
     ap.add_argument("--step-outputs", type=int, default=0, help="server step size (0 = library default)")
     ap.add_argument("--cpu-log2", type=int, default=26)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-comparators", action="store_true", help="skip the rocRAND context timings")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the generator needs no "
@@ -292,11 +300,22 @@ def main():
                 "kernel_ms": round(single["u32"]["kernel_ms"], 4),
                 "gbs": round(single["u32"]["bytes"] / (single["u32"]["kernel_ms"] * 1e-3) / 1e9, 1)},
         "small_2e24_f64": {k: round(v, 4) for k, v in single["small"].items()},
+        "write_ceiling": {"kernel": single["write_ceiling"]["kernel"],
+                          "gbs": round(single["write_ceiling"]["gbs"], 1),
+                          "frac": round(single["write_ceiling"]["gbs"] / PEAK_GBS, 4),
+                          "fill_frac_of_ceiling": round(achieved / single["write_ceiling"]["gbs"], 4)},
     }
     if not args.quick:
         line["batched"] = bench_batched(args)
         line["jumps"] = bench_jumps(args)
         line["served"] = bench_served(args, line["batched"]["gsps"])
+    if not args.quick and not args.no_comparators:
+        from mrg8_amd import comparators
+
+        rates = comparators.rocrand_rates(N_SINGLE)
+        line["rocrand_f64_2e28"] = {k: {kk: round(vv, 4) for kk, vv in v.items()} if "error" not in v else v
+                                    for k, v in rates.items()}
+        log("rocrand: " + ", ".join(f"{k} {v.get('steady_gsps', 0):.1f}" for k, v in rates.items()))
     if not args.no_cpu:
         line.update(cpu_baselines(args))
     print(json.dumps(line), flush=True)

@@ -45,13 +45,9 @@ LaunchConfig launch_config(const mrg8_batch_config& c) {
   return l;
 }
 
-uint64_t grid_waves(const mrg8_batch* b, const mrg8_batch_config& c) {
-  return (uint64_t)b->ctx->num_cu * (uint64_t)c.blocks_per_cu * (uint64_t)(c.block_threads / 64);
-}
-
 uint64_t count_tiles(const BatchSegment* segs, uint32_t n, uint64_t tile) {
   uint64_t t = 0;
-  for (uint32_t i = 0; i < n; ++i) t += (segs[i].count + tile - 1) / tile;
+  for (uint32_t i = 0; i < n; ++i) t += segment_tiles(segs[i].out, segs[i].format, segs[i].count, tile);
   return t;
 }
 
@@ -99,11 +95,11 @@ extern "C" int mrg8_batch_set_config(mrg8_batch_t b, const mrg8_batch_config* cf
   if (rc) return rc;
   const uint32_t n = b->slot.staged_count();
   if (n) {
-    // the tile map and tiles per grab depend on the tile size and grid
+    // the tile map depends on the tile size
     if (count_tiles(b->slot.segs(), n, cfg->tile_outputs) > b->max_tiles)
       return fail(MRG8_ERR_CAPACITY, "staged segments need more than max_tiles tiles at this tile size");
     if ((rc = b->slot.wait_staging()) != MRG8_OK) return rc;
-    b->slot.finalise(n, cfg->tile_outputs, grid_waves(b, *cfg));
+    b->slot.finalise(n, cfg->tile_outputs);
   }
   b->cfg = *cfg;
   return MRG8_OK;
@@ -126,7 +122,7 @@ extern "C" int mrg8_batch_set(mrg8_batch_t b, const mrg8_stream_desc* descs, uin
     if (reinterpret_cast<uintptr_t>(d.out) % (uintptr_t)esz)
       return fail(MRG8_ERR_ARG, "segment %u: output must be %d-byte aligned", i, esz);
     if (d.count > UINT64_MAX - d.offset) return fail(MRG8_ERR_ARG, "segment %u: offset + count overflows", i);
-    tiles += (d.count + b->cfg.tile_outputs - 1) / b->cfg.tile_outputs;
+    tiles += segment_tiles(d.out, d.format, d.count, b->cfg.tile_outputs);
     total += d.count;
   }
   if (tiles > b->max_tiles)
@@ -145,7 +141,7 @@ extern "C" int mrg8_batch_set(mrg8_batch_t b, const mrg8_stream_desc* descs, uin
     s.format = d.format;
     s.stream = kInlineState;
   }
-  b->slot.finalise(count, b->cfg.tile_outputs, grid_waves(b, b->cfg));
+  b->slot.finalise(count, b->cfg.tile_outputs);
   b->total = total;
   return MRG8_OK;
 }
@@ -218,8 +214,8 @@ extern "C" int mrg8_batch_autotune(mrg8_batch_t b, void* stream, const char* cac
   // threads-per-block x waves-per-CU x tile size, the counterpart of the
   // reference's manual TNUM x BS sweep (cuda/tests/test_mrg8_outer.cu:50-53)
   const int blocks[] = {256, 512, 1024};
-  const int waves_per_cu[] = {16, 32};
-  const uint64_t tiles[] = {2048, 4096, 8192, 16384, 65536};
+  const int waves_per_cu[] = {4, 8, 16, 32};
+  const uint64_t tiles[] = {2048, 4096, 8192, 16384};
   double best_ms = 1e30;
   mrg8_batch_config best_cfg = b->cfg;
   int rc = MRG8_OK;

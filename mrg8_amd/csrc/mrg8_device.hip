@@ -151,11 +151,20 @@ __device__ __forceinline__ void store_lane(typename Out<FMT>::T* p, const uint32
   }
 }
 
+// This lane's PL records: row 0 of A^(PL lane + i + 1), i < PL.
+template <int PL>
+__device__ __forceinline__ void load_lane_coefs(Coef (&k)[PL], const CoefRecord* __restrict__ lane_rec) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int i = 0; i < PL; ++i) k[i] = load_coef(lane_rec + PL * lane + i);
+}
+
 // Generates `cnt` outputs of the stream whose wave-uniform state (at the
-// first output) is s, into dst[0..cnt).  `cnt` and s must be wave-uniform.
+// first output) is s, into dst[0..cnt), with this lane's records k.  `cnt`
+// and s must be wave-uniform.
 template <int FMT, int PL>
 __device__ __forceinline__ void generate_run(typename Out<FMT>::T* __restrict__ dst, uint64_t cnt, uint32_t s[8],
-                                             const CoefRecord* __restrict__ lane_rec) {
+                                             const Coef (&k)[PL]) {
   constexpr int kPer = 64 * PL;
   const int lane = threadIdx.x & 63;
   typename Out<FMT>::T* p = dst + PL * lane;
@@ -168,9 +177,6 @@ __device__ __forceinline__ void generate_run(typename Out<FMT>::T* __restrict__ 
     for (uint64_t i = lane; i < cnt; i += 64) __builtin_nontemporal_store(convert_zero<FMT>(), dst + i);
     return;
   }
-  Coef k[PL];
-#pragma unroll
-  for (int i = 0; i < PL; ++i) k[i] = load_coef(lane_rec + PL * lane + i);
   for (uint64_t it = 0; it < full; ++it) {
     int32_t e[8];
     centre(s, e);
@@ -266,7 +272,9 @@ __global__ __launch_bounds__(1024) void k_fill(FillArgs a) {
   uint32_t s[8];
 #pragma unroll
   for (int j = 0; j < 8; ++j) s[j] = __builtin_amdgcn_readfirstlane(start_state[wib][j]);
-  generate_run<FMT, PL>(static_cast<typename Out<FMT>::T*>(a.out) + first, cnt, s, a.lane);
+  Coef k[PL];
+  load_lane_coefs<PL>(k, a.lane);
+  generate_run<FMT, PL>(static_cast<typename Out<FMT>::T*>(a.out) + first, cnt, s, k);
 }
 
 // Outputs needed before `p` is 16-byte aligned (p itself element-aligned).
@@ -275,13 +283,14 @@ __device__ __forceinline__ uint32_t peel_count(const T* p) {
   return (uint32_t)(((16u - (reinterpret_cast<uintptr_t>(p) & 15u)) & 15u) / sizeof(T));
 }
 
-// One tile of a batch segment: peels up to one double (three words) to reach
-// 16-byte alignment, then runs the wide-store loop.
+// One tile of a batch segment at dst: peels up to one double (three words)
+// to reach 16-byte alignment (only tile 0 of a segment can need it), then
+// runs the wide-store loop.
 template <int FMT>
-__device__ __forceinline__ void batch_tile(const BatchSegment& sg, uint64_t first, uint64_t cnt, uint32_t s[8],
+__device__ __forceinline__ void batch_tile(void* out, uint64_t cnt, uint32_t s[8], const Coef (&k)[2],
                                            const CoefRecord* __restrict__ lane_rec) {
   typedef typename Out<FMT>::T T;
-  T* dst = static_cast<T*>(sg.out) + first;
+  T* dst = static_cast<T*>(out);
   uint32_t peel = peel_count(dst);
   if (peel > cnt) peel = (uint32_t)cnt;
   if (peel) {
@@ -289,29 +298,46 @@ __device__ __forceinline__ void batch_tile(const BatchSegment& sg, uint64_t firs
     dst += peel;
     cnt -= peel;
   }
-  if (cnt) generate_run<FMT, 2>(dst, cnt, s, lane_rec);
+  if (cnt) generate_run<FMT, 2>(dst, cnt, s, k);
 }
 
-// Persistent batched kernel.  Tiles are taken in groups of `grab`
-// consecutive tiles; wave w of W handles groups w, w + W, w + 2W, ... (tiles
-// are equal-sized, so a static round robin balances without any atomic).
-// The states of a group's tiles are jumped together (jump_groups), then the
-// tiles are generated one after another.
+__device__ __forceinline__ uint64_t readlane_u64(uint64_t v, int lane) {
+  const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)v, lane);
+  const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(v >> 32), lane);
+  return (uint64_t)lo | ((uint64_t)hi << 32);
+}
+
+// Persistent batched kernel.  With W waves in the grid, wave w takes tiles
+// w, w + W, w + 2W, ...: at any moment the grid writes a window of about W
+// consecutive tiles, as a one-tile-per-wave launch would, and equal tiles
+// keep the waves in step without any atomic.  A wave looks up its next
+// kGroup tiles together (one 8-lane group each: segment, destination, count,
+// start state, which the group then jumps) and generates them one after
+// another, so a tile starts storing without waiting on memory.
+constexpr uint32_t kGroup = 8;
+
 __global__ __launch_bounds__(1024) void k_batch(BatchArgs a) {
   const int lane = threadIdx.x & 63;
   const int grp = lane >> 3;
   const uint64_t ntiles = a.hdr->ntiles;
-  const uint32_t grab = a.hdr->grab;
   const uint64_t nwaves = (uint64_t)gridDim.x * (blockDim.x >> 6);
   const uint64_t wave = (uint64_t)blockIdx.x * (blockDim.x >> 6) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  for (uint64_t base = wave * grab; base < ntiles; base += nwaves * grab) {
-    const uint32_t ngrp = (uint32_t)min<uint64_t>(grab, ntiles - base);
-    // per group: its tile, segment, starting state and offset
+  if (wave >= ntiles) return;
+  Coef k[2];
+  load_lane_coefs<2>(k, a.lane);
+  const uint64_t mine_total = (ntiles - wave + nwaves - 1) / nwaves;
+  for (uint64_t j0 = 0; j0 < mine_total; j0 += kGroup) {
+    const uint32_t ngrp = (uint32_t)min<uint64_t>(kGroup, mine_total - j0);
     const bool mine = (uint32_t)grp < ngrp;
-    const uint64_t t = base + (mine ? grp : 0);
+    const uint64_t t = (j0 + (mine ? grp : 0)) * nwaves + wave;
     const uint32_t seg = a.tile_seg[t];
     const BatchSegment* sp = a.segs + seg;
-    const uint64_t first = (t - a.tile_begin[seg]) * a.tile;
+    const int32_t fmt = sp->format;
+    const uint64_t kt = t - a.tile_begin[seg];  // tile number within the segment
+    const uint64_t head = segment_head(sp->out, fmt, sp->count);
+    const uint64_t first = kt ? head + kt * a.tile : 0;
+    const uint64_t cnt = min(sp->count, head + (kt + 1) * a.tile) - first;
+    const uint64_t dst = reinterpret_cast<uint64_t>(sp->out) + first * (fmt == 2 ? 4u : 8u);
     const uint32_t sid = sp->stream;
     const uint32_t* src = sid == kInlineState ? sp->state : a.states + 8ull * sid;
     uint32_t s[8];
@@ -322,22 +348,15 @@ __global__ __launch_bounds__(1024) void k_batch(BatchArgs a) {
       uint32_t u[8];
 #pragma unroll
       for (int j = 0; j < 8; ++j) u[j] = __builtin_amdgcn_readlane(s[j], 8 * g);
-      const uint32_t useg = __builtin_amdgcn_readlane(seg, 8 * g);
-      const BatchSegment sg = a.segs[useg];
-      const uint64_t ufirst = ((base + g) - a.tile_begin[useg]) * a.tile;
-      const uint64_t cnt = min(a.tile, sg.count - ufirst);
-      switch (__builtin_amdgcn_readfirstlane(sg.format)) {
-        case 0: batch_tile<0>(sg, ufirst, cnt, u, a.lane); break;
-        case 1: batch_tile<1>(sg, ufirst, cnt, u, a.lane); break;
-        default: batch_tile<2>(sg, ufirst, cnt, u, a.lane); break;
+      void* out = reinterpret_cast<void*>(readlane_u64(dst, 8 * g));
+      const uint64_t ucnt = readlane_u64(cnt, 8 * g);
+      switch (__builtin_amdgcn_readlane(fmt, 8 * g)) {
+        case 0: batch_tile<0>(out, ucnt, u, k, a.lane); break;
+        case 1: batch_tile<1>(out, ucnt, u, k, a.lane); break;
+        default: batch_tile<2>(out, ucnt, u, k, a.lane); break;
       }
     }
   }
-}
-
-__global__ void k_zero_u32(uint32_t* p, uint32_t count) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < count) p[i] = 0u;
 }
 
 template <int FMT, int PL>
@@ -369,12 +388,6 @@ hipError_t launch_fill(const FillArgs& a, int format, int per_lane, int block_th
 
 hipError_t launch_batch(const BatchArgs& a, int grid, int block_threads, hipStream_t stream) {
   hipLaunchKernelGGL(k_batch, dim3(grid), dim3(block_threads), 0, stream, a);
-  return hipGetLastError();
-}
-
-hipError_t launch_zero_u32(uint32_t* p, uint32_t count, hipStream_t stream) {
-  if (count == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_zero_u32, dim3((count + 255) / 256), dim3(256), 0, stream, p, count);
   return hipGetLastError();
 }
 

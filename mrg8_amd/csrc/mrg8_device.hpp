@@ -54,11 +54,30 @@ struct BatchSegment {
 };
 static_assert(sizeof(BatchSegment) == 64, "segment layout");
 
+// Tiles of a batch segment: tile 0 covers the outputs before the first
+// 128-byte boundary of `out` (the head) plus `tile` more; tile k >= 1 covers
+// [head + k tile, head + (k + 1) tile), so every later tile starts on a full
+// cache line and needs no alignment peel.  Shared by the host (tile map) and
+// the kernel (tile bounds).
+constexpr uint64_t kTileAlign = 128;
+
+__host__ __device__ inline uint64_t segment_head(const void* out, int32_t format, uint64_t count) {
+  const uint64_t esz = format == 2 ? 4u : 8u;
+  const uint64_t h = ((kTileAlign - (reinterpret_cast<uintptr_t>(out) & (kTileAlign - 1))) & (kTileAlign - 1)) / esz;
+  return h < count ? h : count;
+}
+
+__host__ __device__ inline uint64_t segment_tiles(const void* out, int32_t format, uint64_t count, uint64_t tile) {
+  if (count == 0) return 0;
+  const uint64_t h = segment_head(out, format, count);
+  return count == h ? 1 : (count - h + tile - 1) / tile;
+}
+
 // Device-resident header of a batch: counts change per run without
 // re-capturing the launch.
 struct BatchHeader {
   uint32_t nsegs;
-  uint32_t grab;     // tiles a wave takes per counter increment (1..8)
+  uint32_t reserved;
   uint64_t ntiles;
 };
 
@@ -68,7 +87,6 @@ struct BatchArgs {
   const uint64_t* tile_begin;  // prefix sum of tiles per segment, nsegs + 1 entries
   const uint32_t* tile_seg;    // segment of each tile, ntiles entries
   uint64_t tile;               // outputs per tile
-  uint32_t* cursor;            // work counter, zeroed at the start of the graph
   const uint32_t* states;      // resident per-stream states (8 words each), may be null
   const CoefRecord* lane;
   const CoefRecord* jump;
@@ -77,6 +95,5 @@ struct BatchArgs {
 // Launchers (mrg8_device.hip).  All are asynchronous on `stream`.
 hipError_t launch_fill(const FillArgs& a, int format, int per_lane, int block_threads, hipStream_t stream);
 hipError_t launch_batch(const BatchArgs& a, int grid, int block_threads, hipStream_t stream);
-hipError_t launch_zero_u32(uint32_t* p, uint32_t count, hipStream_t stream);
 
 }  // namespace mrg8

@@ -213,8 +213,7 @@ int launch_steps(mrg8_server* s) {
       }
     }
     if (n == 0) return MRG8_OK;
-    step->slot.finalise(n, s->launch.tile,
-                        (uint64_t)s->ctx->num_cu * s->launch.blocks_per_cu * (s->launch.block_threads / 64));
+    step->slot.finalise(n, s->launch.tile);
     if ((rc = step->slot.run(s->launch, s->stream)) != MRG8_OK) return rc;
     step->busy = true;
     s->inflight.push_back(step);

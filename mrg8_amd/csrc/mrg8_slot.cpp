@@ -10,13 +10,12 @@ namespace {
 size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
 struct Layout {
-  size_t segs, tiles, tile_seg, cursor, total;
+  size_t segs, tiles, tile_seg, total;
   Layout(uint32_t cap, uint64_t max_tiles) {
     segs = align_up(sizeof(BatchHeader), 64);
     tiles = segs + sizeof(BatchSegment) * (size_t)cap;
     tile_seg = align_up(tiles + sizeof(uint64_t) * ((size_t)cap + 1), 64);
-    cursor = align_up(tile_seg + sizeof(uint32_t) * (size_t)max_tiles, 256);
-    total = cursor + 256;
+    total = align_up(tile_seg + sizeof(uint32_t) * (size_t)max_tiles, 256);
   }
 };
 }  // namespace
@@ -30,7 +29,7 @@ int BatchSlot::init(DeviceContext* ctx, uint32_t cap, uint64_t max_tiles, const 
   const Layout l(cap, max_tiles);
   DeviceGuard dg(ctx->device);
   hipError_t e = hipMalloc((void**)&dmem_, l.total);
-  if (e == hipSuccess) e = hipHostMalloc((void**)&hmem_, l.cursor, hipHostMallocDefault);
+  if (e == hipSuccess) e = hipHostMalloc((void**)&hmem_, l.total, hipHostMallocDefault);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&staged_, hipEventDisableTiming);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&run_done_, hipEventDisableTiming);
   if (e != hipSuccess) {
@@ -38,12 +37,11 @@ int BatchSlot::init(DeviceContext* ctx, uint32_t cap, uint64_t max_tiles, const 
     release();
     return rc;
   }
-  std::memset(hmem_, 0, l.cursor);
+  std::memset(hmem_, 0, l.total);
   d_hdr_ = reinterpret_cast<BatchHeader*>(dmem_);
   d_segs_ = reinterpret_cast<BatchSegment*>(dmem_ + l.segs);
   d_tiles_ = reinterpret_cast<uint64_t*>(dmem_ + l.tiles);
   d_tile_seg_ = reinterpret_cast<uint32_t*>(dmem_ + l.tile_seg);
-  d_cursor_ = reinterpret_cast<uint32_t*>(dmem_ + l.cursor);
   h_hdr_ = reinterpret_cast<BatchHeader*>(hmem_);
   h_segs_ = reinterpret_cast<BatchSegment*>(hmem_ + l.segs);
   h_tiles_ = reinterpret_cast<uint64_t*>(hmem_ + l.tiles);
@@ -79,21 +77,16 @@ int BatchSlot::wait_staging() {
   return MRG8_OK;
 }
 
-uint64_t BatchSlot::finalise(uint32_t count, uint64_t tile, uint64_t grid_waves) {
+uint64_t BatchSlot::finalise(uint32_t count, uint64_t tile) {
   uint64_t t = 0;
   for (uint32_t i = 0; i < count; ++i) {
     h_tiles_[i] = t;
-    const uint64_t n = (h_segs_[i].count + tile - 1) / tile;
+    const uint64_t n = segment_tiles(h_segs_[i].out, h_segs_[i].format, h_segs_[i].count, tile);
     for (uint64_t k = 0; k < n && t + k < max_tiles_; ++k) h_tile_seg_[t + k] = i;
     t += n;
   }
   h_tiles_[count] = t;
-  // Group tiles (their jumps then share one pass) only as far as every wave
-  // of the grid still gets a group.
-  uint64_t grab = grid_waves ? t / grid_waves : 1;
-  grab = grab < 1 ? 1 : (grab > 8 ? 8 : grab);
   h_hdr_->nsegs = count;
-  h_hdr_->grab = (uint32_t)grab;
   h_hdr_->ntiles = t;
   count_ = count;
   ntiles_ = t;
@@ -120,17 +113,15 @@ int BatchSlot::capture(const LaunchConfig& cfg) {
   a.tile_begin = d_tiles_;
   a.tile_seg = d_tile_seg_;
   a.tile = cfg.tile;
-  a.cursor = d_cursor_;
   a.states = states_;
   a.lane = ctx_->tables.lane;
   a.jump = ctx_->tables.jump;
   const int grid = ctx_->num_cu * cfg.blocks_per_cu;
   e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
   if (e == hipSuccess) {
-    hipError_t e1 = launch_zero_u32(d_cursor_, 1, cs);  // a kernel, not a memset node
-    hipError_t e2 = launch_batch(a, grid, cfg.block_threads, cs);
-    hipError_t e3 = hipStreamEndCapture(cs, &graph_);
-    e = e1 != hipSuccess ? e1 : (e2 != hipSuccess ? e2 : e3);
+    hipError_t e1 = launch_batch(a, grid, cfg.block_threads, cs);
+    hipError_t e2 = hipStreamEndCapture(cs, &graph_);
+    e = e1 != hipSuccess ? e1 : e2;
   }
   if (e == hipSuccess) e = hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0);
   (void)hipStreamDestroy(cs);

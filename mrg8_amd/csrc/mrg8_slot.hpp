@@ -2,12 +2,12 @@
 //
 // Device block (one allocation):
 //   BatchHeader | BatchSegment[cap] | uint64 tile_begin[cap+1] |
-//   uint32 tile_seg[max_tiles] | uint32 cursor
-// Host side: a pinned staging copy of everything before the cursor.  A run
-// copies the used parts host-to-device when they changed since the last run
-// and replays a captured graph holding [zero the cursor] -> [persistent
-// kernel]; the graph reads the counts from the device header, so it is
-// captured once per launch configuration.
+//   uint32 tile_seg[max_tiles]
+// Host side: a pinned staging copy of the same block.  A run copies the used
+// parts host-to-device when they changed since the last run and replays a
+// captured graph holding the persistent kernel; the kernel reads the counts
+// from the device header, so the graph is captured once per launch
+// configuration.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -42,10 +42,10 @@ class BatchSlot {
   uint64_t max_tiles() const { return max_tiles_; }
   // Waits until the previous run no longer reads the staging block.
   int wait_staging();
-  // Finalises `count` staged segments for tiles of `tile` outputs and a grid
-  // of `grid_waves` waves (prefix sums, tile -> segment map, tiles per
-  // grab).  Returns the number of tiles; the caller keeps it <= max_tiles.
-  uint64_t finalise(uint32_t count, uint64_t tile, uint64_t grid_waves);
+  // Finalises `count` staged segments for tiles of `tile` outputs (prefix
+  // sums, tile -> segment map).  Returns the number of tiles; the caller
+  // keeps it <= max_tiles.
+  uint64_t finalise(uint32_t count, uint64_t tile);
   uint64_t staged_tiles() const { return ntiles_; }
   uint32_t staged_count() const { return count_; }
 
@@ -69,7 +69,6 @@ class BatchSlot {
   BatchSegment* d_segs_ = nullptr;
   uint64_t* d_tiles_ = nullptr;
   uint32_t* d_tile_seg_ = nullptr;
-  uint32_t* d_cursor_ = nullptr;
   BatchHeader* h_hdr_ = nullptr;
   BatchSegment* h_segs_ = nullptr;
   uint64_t* h_tiles_ = nullptr;

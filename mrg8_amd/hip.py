@@ -36,6 +36,7 @@ def _load():
     h.hipMemcpy.argtypes = [vp, vp, sz, ctypes.c_int]
     h.hipMemcpyAsync.argtypes = [vp, vp, sz, ctypes.c_int, vp]
     h.hipMemsetD8.argtypes = [vp, ctypes.c_ubyte, sz]
+    h.hipMemsetAsync.argtypes = [vp, ctypes.c_int, sz, vp]
     h.hipDeviceSynchronize.argtypes = []
     h.hipGetDeviceCount.argtypes = [ctypes.POINTER(ctypes.c_int)]
     h.hipSetDevice.argtypes = [ctypes.c_int]
@@ -77,6 +78,12 @@ def device_count() -> int:
 
 def synchronize() -> None:
     check(_load().hipDeviceSynchronize(), "hipDeviceSynchronize")
+
+
+def memset_async(ptr: int, value: int, nbytes: int, stream=None) -> None:
+    """Stream-ordered byte fill (the runtime's own fill kernel)."""
+    sp = ctypes.c_void_p(stream.handle if isinstance(stream, Stream) else stream)
+    check(_load().hipMemsetAsync(ctypes.c_void_p(int(ptr)), int(value), int(nbytes), sp), "hipMemsetAsync")
 
 
 def mem_info():

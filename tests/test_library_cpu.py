@@ -183,3 +183,15 @@ int main() {
     out = subprocess.run([str(exe)], capture_output=True, text=True)
     assert out.returncode == 0
     assert int(out.stdout) == GOLD["states_after"]["1000"][0]
+
+
+def test_torch_op_registered_and_rejects_cpu_tensors():
+    import torch
+
+    from mrg8_amd import torch_ops
+
+    assert hasattr(torch.ops.mrg8, "fill_")
+    with pytest.raises(ValueError):
+        torch.ops.mrg8.fill_(torch.empty(4, dtype=torch.float64), [1] * 8, 0, torch_ops.F64)
+    with pytest.raises(ValueError):
+        torch_ops.Generator(1, device="cpu")
