@@ -319,7 +319,11 @@ constexpr uint32_t kGroup = 8;
 __global__ __launch_bounds__(1024) void k_batch(BatchArgs a) {
   const int lane = threadIdx.x & 63;
   const int grp = lane >> 3;
-  const uint64_t ntiles = a.hdr->ntiles;
+  const BatchHeader* hdr = reinterpret_cast<const BatchHeader*>(a.block);
+  const BatchSegment* segs = reinterpret_cast<const BatchSegment*>(a.block + sizeof(BatchHeader));
+  const uint64_t* tile_begin = reinterpret_cast<const uint64_t*>(a.block + hdr->tile_begin_off);
+  const uint32_t* tile_seg = reinterpret_cast<const uint32_t*>(a.block + hdr->tile_seg_off);
+  const uint64_t ntiles = hdr->ntiles;
   const uint64_t nwaves = (uint64_t)gridDim.x * (blockDim.x >> 6);
   const uint64_t wave = (uint64_t)blockIdx.x * (blockDim.x >> 6) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   if (wave >= ntiles) return;
@@ -330,10 +334,10 @@ __global__ __launch_bounds__(1024) void k_batch(BatchArgs a) {
     const uint32_t ngrp = (uint32_t)min<uint64_t>(kGroup, mine_total - j0);
     const bool mine = (uint32_t)grp < ngrp;
     const uint64_t t = (j0 + (mine ? grp : 0)) * nwaves + wave;
-    const uint32_t seg = a.tile_seg[t];
-    const BatchSegment* sp = a.segs + seg;
+    const uint32_t seg = tile_seg[t];
+    const BatchSegment* sp = segs + seg;
     const int32_t fmt = sp->format;
-    const uint64_t kt = t - a.tile_begin[seg];  // tile number within the segment
+    const uint64_t kt = t - tile_begin[seg];  // tile number within the segment
     const uint64_t head = segment_head(sp->out, fmt, sp->count);
     const uint64_t first = kt ? head + kt * a.tile : 0;
     const uint64_t cnt = min(sp->count, head + (kt + 1) * a.tile) - first;

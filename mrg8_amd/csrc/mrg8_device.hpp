@@ -73,19 +73,25 @@ __host__ __device__ inline uint64_t segment_tiles(const void* out, int32_t forma
   return count == h ? 1 : (count - h + tile - 1) / tile;
 }
 
-// Device-resident header of a batch: counts change per run without
-// re-capturing the launch.
+// A batch block, copied to the device in one piece per run:
+//   BatchHeader | BatchSegment[nsegs] | uint64 tile_begin[nsegs + 1] |
+//   uint32 tile_seg[ntiles]
+// (tile_begin: prefix sum of tiles per segment; tile_seg: segment of each
+// tile).  The header gives the counts and where the two tables start, so the
+// launch reads everything through one fixed pointer and the graph holding it
+// never needs re-capturing.
 struct BatchHeader {
   uint32_t nsegs;
   uint32_t reserved;
   uint64_t ntiles;
+  uint64_t tile_begin_off;  // bytes from the block start
+  uint64_t tile_seg_off;
+  uint64_t pad[4];
 };
+static_assert(sizeof(BatchHeader) == 64, "header layout");
 
 struct BatchArgs {
-  const BatchHeader* hdr;
-  const BatchSegment* segs;
-  const uint64_t* tile_begin;  // prefix sum of tiles per segment, nsegs + 1 entries
-  const uint32_t* tile_seg;    // segment of each tile, ntiles entries
+  const uint8_t* block;
   uint64_t tile;               // outputs per tile
   const uint32_t* states;      // resident per-stream states (8 words each), may be null
   const CoefRecord* lane;

@@ -11,10 +11,12 @@
 //               complete when the step holding its last chunk finishes.
 //   released -> its arena bytes return to the pool.
 // A step is one persistent batched launch (BatchSlot) over chunks of the
-// active requests, filled in admission order up to `step_outputs` outputs
-// (at most a quarter of a step per request, so requests queued behind a long
-// one still start in the next step).  Up to kSlots steps are in flight on
-// the server's stream; a request arriving while they run joins the next one.
+// active requests, up to `step_outputs` outputs and at most a quarter of a
+// step per request.  Service order within a step: requests admitted more
+// than kAgeLimitS ago first (admission order), then the rest by fewest
+// outputs left, so small requests do not queue behind large ones and large
+// ones cannot starve.  Up to kSlots steps are in flight on the server's
+// stream; a request arriving while they run joins the next one.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -36,6 +38,7 @@ using namespace mrg8;
 namespace {
 
 constexpr int kSlots = 3;
+constexpr double kAgeLimitS = 2e-3;
 constexpr size_t kAlign = 256;
 
 double wall_now() {
@@ -101,7 +104,7 @@ struct Request {
   char* out = nullptr;
   uint64_t issued = 0;
   RState st = RState::Queued;
-  double t_submit = 0, t_complete = 0;
+  double t_submit = 0, t_admit = 0, t_complete = 0;
 };
 
 struct Step {
@@ -116,7 +119,8 @@ struct mrg8_server {
   DeviceContext* ctx = nullptr;
   mrg8_server_config cfg{};
   LaunchConfig launch;
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;       // launches
+  hipStream_t copy_stream = nullptr;  // step staging copies, overlapping the previous launch
   uint32_t* d_states = nullptr;
   std::vector<uint32_t> h_states;
   std::vector<uint64_t> cursor;
@@ -125,6 +129,7 @@ struct mrg8_server {
   Arena arena;
   std::unordered_map<uint64_t, std::unique_ptr<Request>> reqs;
   std::deque<Request*> queued, active;
+  std::vector<Request*> order;  // scratch: service order of one step
   std::deque<Request*> done_unreported;
   std::deque<Step*> inflight;  // launch order
   std::vector<std::unique_ptr<Step>> steps;
@@ -166,7 +171,8 @@ void admit(mrg8_server* s) {
     r->arena_off = off;
     r->out = s->d_arena + off;
     r->st = RState::Active;
-    s->counters.max_queue_s = std::max(s->counters.max_queue_s, wall_now() - r->t_submit);
+    r->t_admit = wall_now();
+    s->counters.max_queue_s = std::max(s->counters.max_queue_s, r->t_admit - r->t_submit);
     s->queued.pop_front();
     s->active.push_back(r);
     ++s->n_admitted;
@@ -190,7 +196,14 @@ int launch_steps(mrg8_server* s) {
     uint64_t budget = s->cfg.step_outputs;
     const uint64_t per_req = std::max<uint64_t>(s->launch.tile, s->cfg.step_outputs / 4);
     uint32_t n = 0;
-    for (auto it = s->active.begin(); it != s->active.end() && budget > 0 && n < cap;) {
+    std::vector<Request*>& order = s->order;
+    order.assign(s->active.begin(), s->active.end());
+    const double now = wall_now();
+    auto mid = std::stable_partition(order.begin(), order.end(),
+                                     [&](const Request* r) { return now - r->t_admit > kAgeLimitS; });
+    std::stable_sort(mid, order.end(),
+                     [](const Request* a, const Request* b) { return a->count - a->issued < b->count - b->issued; });
+    for (auto it = order.begin(); it != order.end() && budget > 0 && n < cap; ++it) {
       Request* r = *it;
       const uint64_t left = r->count - r->issued;
       uint64_t take = std::min(left, std::min(budget, per_req));
@@ -207,14 +220,14 @@ int launch_steps(mrg8_server* s) {
       if (r->issued == r->count) {
         r->st = RState::Issued;
         step->finishing.push_back(r->id);
-        it = s->active.erase(it);
-      } else {
-        ++it;
       }
     }
+    s->active.erase(std::remove_if(s->active.begin(), s->active.end(),
+                                   [](const Request* r) { return r->st == RState::Issued; }),
+                    s->active.end());
     if (n == 0) return MRG8_OK;
     step->slot.finalise(n, s->launch.tile);
-    if ((rc = step->slot.run(s->launch, s->stream)) != MRG8_OK) return rc;
+    if ((rc = step->slot.run(s->launch, s->stream, s->copy_stream)) != MRG8_OK) return rc;
     step->busy = true;
     s->inflight.push_back(step);
     s->counters.steps += 1;
@@ -267,6 +280,7 @@ extern "C" int mrg8_server_create(int device, const mrg8_server_config* cfg, uin
   s->pending_per_stream.assign(nst, 0);
   for (size_t i = 0; i < nst; ++i) seed_state(base_seed + (uint32_t)i, &s->h_states[i * 8]);
   hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->copy_stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipMalloc((void**)&s->d_states, nst * 8 * sizeof(uint32_t));
   if (e == hipSuccess)
     e = hipMemcpy(s->d_states, s->h_states.data(), nst * 8 * sizeof(uint32_t), hipMemcpyHostToDevice);
@@ -297,10 +311,12 @@ extern "C" int mrg8_server_destroy(mrg8_server_t s) {
   if (!s) return MRG8_OK;
   DeviceGuard dg(s->ctx ? s->ctx->device : 0);
   if (s->stream) (void)hipStreamSynchronize(s->stream);
+  if (s->copy_stream) (void)hipStreamSynchronize(s->copy_stream);
   s->steps.clear();  // slots wait for their last run
   if (s->d_arena) (void)hipFree(s->d_arena);
   if (s->d_states) (void)hipFree(s->d_states);
   if (s->stream) (void)hipStreamDestroy(s->stream);
+  if (s->copy_stream) (void)hipStreamDestroy(s->copy_stream);
   delete s;
   return MRG8_OK;
 }

@@ -1,5 +1,6 @@
 #include "mrg8_slot.hpp"
 
+#include <algorithm>
 #include <cstring>
 
 #include "../../include/mrg8.h"
@@ -9,15 +10,9 @@ namespace mrg8 {
 namespace {
 size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
-struct Layout {
-  size_t segs, tiles, tile_seg, total;
-  Layout(uint32_t cap, uint64_t max_tiles) {
-    segs = align_up(sizeof(BatchHeader), 64);
-    tiles = segs + sizeof(BatchSegment) * (size_t)cap;
-    tile_seg = align_up(tiles + sizeof(uint64_t) * ((size_t)cap + 1), 64);
-    total = align_up(tile_seg + sizeof(uint32_t) * (size_t)max_tiles, 256);
-  }
-};
+// Byte offsets of the tables behind `n` segments (see mrg8_device.hpp).
+size_t tile_begin_offset(uint32_t n) { return sizeof(BatchHeader) + sizeof(BatchSegment) * (size_t)n; }
+size_t tile_seg_offset(uint32_t n) { return tile_begin_offset(n) + sizeof(uint64_t) * ((size_t)n + 1); }
 }  // namespace
 
 int BatchSlot::init(DeviceContext* ctx, uint32_t cap, uint64_t max_tiles, const uint32_t* states) {
@@ -26,10 +21,10 @@ int BatchSlot::init(DeviceContext* ctx, uint32_t cap, uint64_t max_tiles, const 
   cap_ = cap;
   max_tiles_ = max_tiles;
   states_ = states;
-  const Layout l(cap, max_tiles);
+  const size_t total = align_up(tile_seg_offset(cap) + sizeof(uint32_t) * (size_t)max_tiles, 256);
   DeviceGuard dg(ctx->device);
-  hipError_t e = hipMalloc((void**)&dmem_, l.total);
-  if (e == hipSuccess) e = hipHostMalloc((void**)&hmem_, l.total, hipHostMallocDefault);
+  hipError_t e = hipMalloc((void**)&dmem_, total);
+  if (e == hipSuccess) e = hipHostMalloc((void**)&hmem_, total, hipHostMallocDefault);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&staged_, hipEventDisableTiming);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&run_done_, hipEventDisableTiming);
   if (e != hipSuccess) {
@@ -37,15 +32,9 @@ int BatchSlot::init(DeviceContext* ctx, uint32_t cap, uint64_t max_tiles, const 
     release();
     return rc;
   }
-  std::memset(hmem_, 0, l.total);
-  d_hdr_ = reinterpret_cast<BatchHeader*>(dmem_);
-  d_segs_ = reinterpret_cast<BatchSegment*>(dmem_ + l.segs);
-  d_tiles_ = reinterpret_cast<uint64_t*>(dmem_ + l.tiles);
-  d_tile_seg_ = reinterpret_cast<uint32_t*>(dmem_ + l.tile_seg);
+  std::memset(hmem_, 0, total);
   h_hdr_ = reinterpret_cast<BatchHeader*>(hmem_);
-  h_segs_ = reinterpret_cast<BatchSegment*>(hmem_ + l.segs);
-  h_tiles_ = reinterpret_cast<uint64_t*>(hmem_ + l.tiles);
-  h_tile_seg_ = reinterpret_cast<uint32_t*>(hmem_ + l.tile_seg);
+  h_segs_ = reinterpret_cast<BatchSegment*>(hmem_ + sizeof(BatchHeader));
   return MRG8_OK;
 }
 
@@ -78,16 +67,23 @@ int BatchSlot::wait_staging() {
 }
 
 uint64_t BatchSlot::finalise(uint32_t count, uint64_t tile) {
+  const size_t tb_off = tile_begin_offset(count), ts_off = tile_seg_offset(count);
+  uint64_t* tiles = reinterpret_cast<uint64_t*>(hmem_ + tb_off);
+  uint32_t* tile_seg = reinterpret_cast<uint32_t*>(hmem_ + ts_off);
   uint64_t t = 0;
   for (uint32_t i = 0; i < count; ++i) {
-    h_tiles_[i] = t;
+    tiles[i] = t;
     const uint64_t n = segment_tiles(h_segs_[i].out, h_segs_[i].format, h_segs_[i].count, tile);
-    for (uint64_t k = 0; k < n && t + k < max_tiles_; ++k) h_tile_seg_[t + k] = i;
+    const uint64_t end = std::min<uint64_t>(t + n, max_tiles_);
+    if (t < end) std::fill(tile_seg + t, tile_seg + end, i);
     t += n;
   }
-  h_tiles_[count] = t;
+  tiles[count] = t;
   h_hdr_->nsegs = count;
   h_hdr_->ntiles = t;
+  h_hdr_->tile_begin_off = tb_off;
+  h_hdr_->tile_seg_off = ts_off;
+  used_bytes_ = ts_off + sizeof(uint32_t) * (size_t)std::min<uint64_t>(t, max_tiles_);
   count_ = count;
   ntiles_ = t;
   dirty_ = true;
@@ -108,10 +104,7 @@ int BatchSlot::capture(const LaunchConfig& cfg) {
   hipError_t e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
   if (e != hipSuccess) return hip_fail(e, "hipStreamCreate(capture)");
   BatchArgs a{};
-  a.hdr = d_hdr_;
-  a.segs = d_segs_;
-  a.tile_begin = d_tiles_;
-  a.tile_seg = d_tile_seg_;
+  a.block = dmem_;
   a.tile = cfg.tile;
   a.states = states_;
   a.lane = ctx_->tables.lane;
@@ -131,7 +124,7 @@ int BatchSlot::capture(const LaunchConfig& cfg) {
   return MRG8_OK;
 }
 
-int BatchSlot::run(const LaunchConfig& cfg, hipStream_t stream) {
+int BatchSlot::run(const LaunchConfig& cfg, hipStream_t stream, hipStream_t copy_stream) {
   if (count_ == 0 || ntiles_ == 0) return MRG8_OK;
   if (ntiles_ > max_tiles_) return fail(MRG8_ERR_CAPACITY, "staged work needs more than max_tiles tiles");
   DeviceGuard dg(ctx_->device);
@@ -139,14 +132,13 @@ int BatchSlot::run(const LaunchConfig& cfg, hipStream_t stream) {
   if (rc) return rc;
   hipError_t e = hipSuccess;
   if (dirty_) {  // replays of an unchanged batch copy nothing
-    e = hipMemcpyAsync(d_hdr_, h_hdr_, sizeof(BatchHeader), hipMemcpyHostToDevice, stream);
-    if (e == hipSuccess)
-      e = hipMemcpyAsync(d_segs_, h_segs_, sizeof(BatchSegment) * count_, hipMemcpyHostToDevice, stream);
-    if (e == hipSuccess)
-      e = hipMemcpyAsync(d_tiles_, h_tiles_, sizeof(uint64_t) * (count_ + 1), hipMemcpyHostToDevice, stream);
-    if (e == hipSuccess)
-      e = hipMemcpyAsync(d_tile_seg_, h_tile_seg_, sizeof(uint32_t) * ntiles_, hipMemcpyHostToDevice, stream);
-    if (e == hipSuccess) e = hipEventRecord(staged_, stream);
+    // The device block may be overwritten once this slot's previous launch
+    // has finished: order the copy after it (a no-op on the launch stream).
+    hipStream_t cs = copy_stream ? copy_stream : stream;
+    if (run_pending_ && cs != stream) e = hipStreamWaitEvent(cs, run_done_, 0);
+    if (e == hipSuccess) e = hipMemcpyAsync(dmem_, hmem_, used_bytes_, hipMemcpyHostToDevice, cs);
+    if (e == hipSuccess) e = hipEventRecord(staged_, cs);
+    if (e == hipSuccess && copy_stream) e = hipStreamWaitEvent(stream, staged_, 0);
     if (e != hipSuccess) return hip_fail(e, "batch staging copy");
     copy_pending_ = true;
     dirty_ = false;

@@ -1,13 +1,13 @@
 // A batch slot: everything one in-flight persistent launch needs.
 //
-// Device block (one allocation):
-//   BatchHeader | BatchSegment[cap] | uint64 tile_begin[cap+1] |
-//   uint32 tile_seg[max_tiles]
-// Host side: a pinned staging copy of the same block.  A run copies the used
-// parts host-to-device when they changed since the last run and replays a
-// captured graph holding the persistent kernel; the kernel reads the counts
-// from the device header, so the graph is captured once per launch
-// configuration.
+// One device block holds the batch block of mrg8_device.hpp (header,
+// segments, tile prefix sums, tile -> segment map), sized for `cap` segments
+// and `max_tiles` tiles; a pinned host block of the same size stages it.  A
+// run copies the used prefix host-to-device in one copy when it changed since
+// the last run (optionally on a separate copy stream, so that it overlaps
+// the previous launch) and replays a captured graph holding the persistent
+// kernel; the kernel reads the counts and table offsets from the header, so
+// the graph is captured once per launch configuration.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -49,8 +49,9 @@ class BatchSlot {
   uint64_t staged_tiles() const { return ntiles_; }
   uint32_t staged_count() const { return count_; }
 
-  // Copies the staged block and replays the launch on `stream`.
-  int run(const LaunchConfig& cfg, hipStream_t stream);
+  // Copies the staged block (on `copy_stream` when given, else on `stream`)
+  // and replays the launch on `stream`.
+  int run(const LaunchConfig& cfg, hipStream_t stream, hipStream_t copy_stream = nullptr);
   // The last run finished (non-blocking query).
   bool done();
   int wait_done();
@@ -63,16 +64,11 @@ class BatchSlot {
   uint32_t cap_ = 0;
   uint64_t max_tiles_ = 0;
   const uint32_t* states_ = nullptr;
-  char* dmem_ = nullptr;
-  char* hmem_ = nullptr;
-  BatchHeader* d_hdr_ = nullptr;
-  BatchSegment* d_segs_ = nullptr;
-  uint64_t* d_tiles_ = nullptr;
-  uint32_t* d_tile_seg_ = nullptr;
+  uint8_t* dmem_ = nullptr;
+  uint8_t* hmem_ = nullptr;
+  size_t used_bytes_ = 0;  // staged prefix to copy
   BatchHeader* h_hdr_ = nullptr;
   BatchSegment* h_segs_ = nullptr;
-  uint64_t* h_tiles_ = nullptr;
-  uint32_t* h_tile_seg_ = nullptr;
   hipEvent_t staged_ = nullptr, run_done_ = nullptr;
   bool copy_pending_ = false, run_pending_ = false, dirty_ = false;
   uint32_t count_ = 0;
