@@ -187,7 +187,7 @@ typedef struct mrg8_stream_desc {
 typedef struct mrg8_batch_config {
   int32_t block_threads;  /* 64..1024, multiple of 64 */
   int32_t blocks_per_cu;  /* persistent grid = CUs * blocks_per_cu */
-  uint64_t tile_outputs;  /* outputs per work item, multiple of 64 */
+  uint64_t tile_outputs;  /* outputs per work item, multiple of 256 */
 } mrg8_batch_config;
 
 /* A batch object owns device descriptor and tile tables sized for
@@ -217,6 +217,9 @@ typedef struct mrg8_server_config {
   uint64_t tile_outputs;       /* outputs per work item (0 = automatic) */
   int32_t block_threads;       /* 0 = automatic */
   int32_t blocks_per_cu;       /* 0 = automatic */
+  uint32_t steps_in_flight;    /* launches in flight at once, 1..8 (0 = 2) */
+  uint32_t age_limit_us;       /* requests admitted longer ago than this are served first,
+                                  in admission order; the rest shortest-remaining first (0 = 1000) */
 } mrg8_server_config;
 
 typedef struct mrg8_completion {

@@ -257,9 +257,9 @@ class Server:
 
     def __init__(self, num_streams: int, max_output_bytes: int, base_seed: int = 1, max_inflight: int = 4096,
                  step_outputs: int = 0, tile_outputs: int = 0, block_threads: int = 0, blocks_per_cu: int = 0,
-                 device: int = 0):
+                 device: int = 0, steps_in_flight: int = 0, age_limit_us: int = 0):
         cfg = capi.ServerConfig(num_streams, max_inflight, max_output_bytes, step_outputs, tile_outputs,
-                                block_threads, blocks_per_cu)
+                                block_threads, blocks_per_cu, steps_in_flight, age_limit_us)
         h = ctypes.c_void_p()
         capi.call("mrg8_server_create", device, ctypes.byref(cfg), ctypes.c_uint32(base_seed), ctypes.byref(h))
         self._h = h

@@ -35,7 +35,8 @@ class ServerConfig(ctypes.Structure):
     _fields_ = [("num_streams", ctypes.c_uint32), ("max_inflight", ctypes.c_uint32),
                 ("max_output_bytes", ctypes.c_uint64), ("step_outputs", ctypes.c_uint64),
                 ("tile_outputs", ctypes.c_uint64), ("block_threads", ctypes.c_int32),
-                ("blocks_per_cu", ctypes.c_int32)]
+                ("blocks_per_cu", ctypes.c_int32), ("steps_in_flight", ctypes.c_uint32),
+                ("age_limit_us", ctypes.c_uint32)]
 
 
 class Completion(ctypes.Structure):

@@ -13,10 +13,13 @@
 // A step is one persistent batched launch (BatchSlot) over chunks of the
 // active requests, up to `step_outputs` outputs and at most a quarter of a
 // step per request.  Service order within a step: requests admitted more
-// than kAgeLimitS ago first (admission order), then the rest by fewest
+// than `age_limit_us` ago first (admission order), then the rest by fewest
 // outputs left, so small requests do not queue behind large ones and large
-// ones cannot starve.  Up to kSlots steps are in flight on the server's
-// stream; a request arriving while they run joins the next one.
+// ones cannot starve.  Up to `steps_in_flight` steps are in flight, each on
+// its own stream so that a step starts while the previous one's last waves
+// drain; a request arriving while they run joins the next one.  Each step's
+// staging copy runs on a separate copy stream, overlapping the launches
+// before it.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -37,8 +40,7 @@ using namespace mrg8;
 
 namespace {
 
-constexpr int kSlots = 3;
-constexpr double kAgeLimitS = 2e-3;
+constexpr uint32_t kMaxSteps = 8;
 constexpr size_t kAlign = 256;
 
 double wall_now() {
@@ -109,6 +111,7 @@ struct Request {
 
 struct Step {
   BatchSlot slot;
+  hipStream_t stream = nullptr;  // this step's launches: the next step can start during this one's tail
   std::vector<uint64_t> finishing;  // requests whose last chunk is in this step
   bool busy = false;
 };
@@ -119,8 +122,7 @@ struct mrg8_server {
   DeviceContext* ctx = nullptr;
   mrg8_server_config cfg{};
   LaunchConfig launch;
-  hipStream_t stream = nullptr;       // launches
-  hipStream_t copy_stream = nullptr;  // step staging copies, overlapping the previous launch
+  hipStream_t copy_stream = nullptr;  // step staging copies, overlapping the launches before
   uint32_t* d_states = nullptr;
   std::vector<uint32_t> h_states;
   std::vector<uint64_t> cursor;
@@ -199,8 +201,9 @@ int launch_steps(mrg8_server* s) {
     std::vector<Request*>& order = s->order;
     order.assign(s->active.begin(), s->active.end());
     const double now = wall_now();
+    const double age_limit = 1e-6 * s->cfg.age_limit_us;
     auto mid = std::stable_partition(order.begin(), order.end(),
-                                     [&](const Request* r) { return now - r->t_admit > kAgeLimitS; });
+                                     [&](const Request* r) { return now - r->t_admit > age_limit; });
     std::stable_sort(mid, order.end(),
                      [](const Request* a, const Request* b) { return a->count - a->issued < b->count - b->issued; });
     for (auto it = order.begin(); it != order.end() && budget > 0 && n < cap; ++it) {
@@ -227,7 +230,7 @@ int launch_steps(mrg8_server* s) {
                     s->active.end());
     if (n == 0) return MRG8_OK;
     step->slot.finalise(n, s->launch.tile);
-    if ((rc = step->slot.run(s->launch, s->stream, s->copy_stream)) != MRG8_OK) return rc;
+    if ((rc = step->slot.run(s->launch, step->stream, s->copy_stream)) != MRG8_OK) return rc;
     step->busy = true;
     s->inflight.push_back(step);
     s->counters.steps += 1;
@@ -260,12 +263,19 @@ extern "C" int mrg8_server_create(int device, const mrg8_server_config* cfg, uin
   s->ctx = ctx;
   s->cfg = *cfg;
   if (!s->cfg.max_inflight) s->cfg.max_inflight = 4096;
-  // A step should give every wave of the persistent grid several tiles:
-  // 4096-output tiles, 2^27-output steps (32768 tiles for 8192 waves).
+  // Defaults measured on MI355X with the log-uniform request mix at 490-600
+  // G samples/s offered: 4096-output tiles, one 256-thread block per CU (the
+  // batched kernel's best shape), 2^27-output steps (32 tiles per wave),
+  // two steps in flight, 1 ms age limit.  Smaller steps cut the median but
+  // stop keeping up above ~550 G samples/s; more steps in flight only add
+  // queueing in front of a new request.
   if (!s->cfg.tile_outputs) s->cfg.tile_outputs = 4096;
   if (!s->cfg.step_outputs) s->cfg.step_outputs = 1ull << 27;
-  if (!s->cfg.block_threads) s->cfg.block_threads = 1024;
-  if (!s->cfg.blocks_per_cu) s->cfg.blocks_per_cu = 2;
+  if (!s->cfg.block_threads) s->cfg.block_threads = 256;
+  if (!s->cfg.blocks_per_cu) s->cfg.blocks_per_cu = 1;
+  if (!s->cfg.steps_in_flight) s->cfg.steps_in_flight = 2;
+  if (!s->cfg.age_limit_us) s->cfg.age_limit_us = 1000;
+  if (s->cfg.steps_in_flight > kMaxSteps) return fail(MRG8_ERR_ARG, "steps_in_flight must be at most %u", kMaxSteps);
   if (s->cfg.tile_outputs % 256 || s->cfg.block_threads % 64 || s->cfg.block_threads > 1024 ||
       s->cfg.blocks_per_cu * (s->cfg.block_threads / 64) > 32)
     return fail(MRG8_ERR_ARG, "invalid launch configuration");
@@ -279,8 +289,7 @@ extern "C" int mrg8_server_create(int device, const mrg8_server_config* cfg, uin
   s->cursor.assign(nst, 0);
   s->pending_per_stream.assign(nst, 0);
   for (size_t i = 0; i < nst; ++i) seed_state(base_seed + (uint32_t)i, &s->h_states[i * 8]);
-  hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
-  if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->copy_stream, hipStreamNonBlocking);
+  hipError_t e = hipStreamCreateWithFlags(&s->copy_stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipMalloc((void**)&s->d_states, nst * 8 * sizeof(uint32_t));
   if (e == hipSuccess)
     e = hipMemcpy(s->d_states, s->h_states.data(), nst * 8 * sizeof(uint32_t), hipMemcpyHostToDevice);
@@ -294,13 +303,20 @@ extern "C" int mrg8_server_create(int device, const mrg8_server_config* cfg, uin
   // Segments per step: every admitted request may contribute one chunk.
   const uint32_t seg_cap = std::min<uint32_t>(s->cfg.max_inflight, 1u << 16);
   const uint64_t tile_cap = s->cfg.step_outputs / s->cfg.tile_outputs + seg_cap + 1;
-  for (int i = 0; i < kSlots; ++i) {
+  for (uint32_t i = 0; i < s->cfg.steps_in_flight; ++i) {
     auto st = std::make_unique<Step>();
-    if ((rc = st->slot.init(ctx, seg_cap, tile_cap, s->d_states)) != MRG8_OK) {
+    e = hipStreamCreateWithFlags(&st->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+      rc = hip_fail(e, "server stream");
       mrg8_server_destroy(s.release());
       return rc;
     }
+    const bool ok = (rc = st->slot.init(ctx, seg_cap, tile_cap, s->d_states)) == MRG8_OK;
     s->steps.push_back(std::move(st));
+    if (!ok) {
+      mrg8_server_destroy(s.release());
+      return rc;
+    }
   }
   *out = s.release();
   return MRG8_OK;
@@ -310,12 +326,16 @@ extern "C" int mrg8_server_destroy(mrg8_server_t s) {
   clear_error();
   if (!s) return MRG8_OK;
   DeviceGuard dg(s->ctx ? s->ctx->device : 0);
-  if (s->stream) (void)hipStreamSynchronize(s->stream);
+  for (auto& st : s->steps)
+    if (st->stream) (void)hipStreamSynchronize(st->stream);
   if (s->copy_stream) (void)hipStreamSynchronize(s->copy_stream);
-  s->steps.clear();  // slots wait for their last run
+  for (auto& st : s->steps) {
+    st->slot.release();  // waits for its last run
+    if (st->stream) (void)hipStreamDestroy(st->stream);
+  }
+  s->steps.clear();
   if (s->d_arena) (void)hipFree(s->d_arena);
   if (s->d_states) (void)hipFree(s->d_states);
-  if (s->stream) (void)hipStreamDestroy(s->stream);
   if (s->copy_stream) (void)hipStreamDestroy(s->copy_stream);
   delete s;
   return MRG8_OK;
@@ -331,8 +351,10 @@ extern "C" int mrg8_server_set_stream(mrg8_server_t s, uint32_t stream_id, const
   DeviceGuard dg(s->ctx->device);
   std::memcpy(&s->h_states[(size_t)stream_id * 8], state, 8 * sizeof(uint32_t));
   s->cursor[stream_id] = position;
-  // in-flight steps of other streams read the table: copy in stream order
-  hipError_t e = hipStreamSynchronize(s->stream);
+  // in-flight steps of other streams read the table: copy once they finish
+  hipError_t e = hipSuccess;
+  for (auto& st : s->steps)
+    if (e == hipSuccess) e = hipStreamSynchronize(st->stream);
   if (e == hipSuccess)
     e = hipMemcpy(s->d_states + (size_t)stream_id * 8, state, 8 * sizeof(uint32_t), hipMemcpyHostToDevice);
   if (e != hipSuccess) return hip_fail(e, "state upload");
