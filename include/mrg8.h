@@ -183,7 +183,8 @@ typedef struct mrg8_stream_desc {
   uint32_t reserved; /* must be 0 */
 } mrg8_stream_desc;
 
-/* Launch configuration (picked by mrg8_batch_autotune or set by hand). */
+/* Launch configuration (picked by mrg8_batch_autotune or set by hand; a new
+ * batch starts with 256 threads, 1 block per CU, 4096-output tiles). */
 typedef struct mrg8_batch_config {
   int32_t block_threads;  /* 64..1024, multiple of 64 */
   int32_t blocks_per_cu;  /* persistent grid = CUs * blocks_per_cu */

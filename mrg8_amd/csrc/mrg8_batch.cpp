@@ -22,7 +22,9 @@ struct mrg8_batch {
   uint64_t max_tiles = 0;
   BatchSlot slot;
   uint64_t total = 0;  // outputs staged
-  mrg8_batch_config cfg{256, 8, 65536};
+  // one 256-thread block per CU, 4096-output tiles: the autotuner's pick for
+  // thousands of streams of uneven length on MI355X
+  mrg8_batch_config cfg{256, 1, 4096};
 };
 
 namespace {

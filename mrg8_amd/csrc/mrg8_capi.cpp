@@ -143,13 +143,13 @@ int pick_per_lane(int format, const void* p) {
 }
 
 // Default launch shape for a single-stream fill of n outputs.  Measured on
-// MI355X (2^28 outputs): doubles are fastest with 1024-thread blocks and 4096
+// MI355X (2^28 outputs): doubles are fastest with 1024-thread blocks and 2048
 // outputs per wave, words with 256-thread blocks and 8192 per wave; small
 // fills shrink the per-wave run so that every SIMD still gets waves.
 void default_shape(const DeviceContext& ctx, uint64_t n, int format, int per_lane, int* block, uint64_t* tile) {
   const uint64_t quantum = 64ull * per_lane;
   *block = format == MRG8_U32 ? 256 : 1024;
-  uint64_t t = format == MRG8_U32 ? 8192 : 4096;
+  uint64_t t = format == MRG8_U32 ? 8192 : 2048;
   const uint64_t min_waves = (uint64_t)ctx.num_cu * 4 * 4;  // four waves per SIMD
   while (t > 16 * quantum && (n + t - 1) / t < min_waves) t /= 2;
   *tile = std::max<uint64_t>(quantum, t / quantum * quantum);

@@ -61,6 +61,26 @@ __device__ __forceinline__ uint32_t dot_residue(const Coef& k, const int32_t e[8
   return min(x, x - kP);  // x - p wraps above x when x < p
 }
 
+// PL residues at once, the eight multiply-adds of the PL dot products
+// interleaved so that consecutive 64-bit multiply-adds are independent and
+// issue back to back.
+template <int PL>
+__device__ __forceinline__ void dot_residues(const Coef (&k)[PL], const int32_t e[8], uint32_t r[PL]) {
+  uint64_t v[PL];
+#pragma unroll
+  for (int i = 0; i < PL; ++i) v[i] = k[i].b0;
+#pragma unroll
+  for (int j = 0; j < 8; ++j)
+#pragma unroll
+    for (int i = 0; i < PL; ++i) v[i] += (uint64_t)((int64_t)k[i].c[j] * (int64_t)e[j]);
+#pragma unroll
+  for (int i = 0; i < PL; ++i) {
+    const uint64_t w = fold_hi((uint32_t)(v[i] >> 32), (uint32_t)v[i]);
+    const uint32_t x = ((uint32_t)w & kP) + (uint32_t)(w >> 31);
+    r[i] = min(x, x - kP);
+  }
+}
+
 __device__ __forceinline__ void centre(const uint32_t s[8], int32_t e[8]) {
 #pragma unroll
   for (int j = 0; j < 8; ++j) e[j] = (int32_t)(s[j] - kCenter);
@@ -82,7 +102,7 @@ __device__ __forceinline__ uint64_t drop_digit(uint64_t off) {
   return off & ~(15ull << (__builtin_ctzll(off) & ~3));
 }
 
-__device__ void jump_groups(uint32_t s[8], uint64_t off, const CoefRecord* __restrict__ jrec) {
+__device__ __forceinline__ void jump_groups(uint32_t s[8], uint64_t off, const CoefRecord* __restrict__ jrec) {
   const int lane = threadIdx.x & 63;
   const int grp = lane >> 3, row = lane & 7;
   Coef k = {};
@@ -181,8 +201,7 @@ __device__ __forceinline__ void generate_run(typename Out<FMT>::T* __restrict__ 
     int32_t e[8];
     centre(s, e);
     uint32_t r[PL];
-#pragma unroll
-    for (int i = 0; i < PL; ++i) r[i] = dot_residue(k[i], e);
+    dot_residues<PL>(k, e, r);
     store_lane<FMT, PL>(p, r);
     p += kPer;
     // next state: the last eight words, newest first

@@ -4,12 +4,14 @@
     rocprofv3 --pmc FETCH_SIZE -d OUT -o fetch -- python3 -m mrg8_amd.profile_workload
     rocprofv3 --pmc WRITE_SIZE -d OUT -o write -- python3 -m mrg8_amd.profile_workload
 
-Dispatches, in order (each `--reps` times):
+Dispatches, in order (each `--warmup` + `--reps` times; summaries use the
+last `--reps` dispatches of each kind):
   1. a runtime device-to-device copy of 1 GiB  -- the calibration kernel of
      known traffic (1 GiB read + 1 GiB written)
   2. single-stream fill, 2^28 float64 outputs (seed 13579)
   3. single-stream fill, 2^28 uint32 outputs
-  4. batched launch: 4096 streams, log-uniform lengths, float64   (--batched)
+  4. batched launch: 4096 streams, log-uniform lengths, float64, 256 threads
+     x 1 block per CU, 4096-output tiles                          (--batched)
   5. jump-heavy batched launch: 65536 x (63-bit jump + 256 uint32) (--batched)
 """
 from __future__ import annotations
@@ -26,6 +28,7 @@ sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--reps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--batched", action="store_true")
     args = ap.parse_args()
 
@@ -36,7 +39,7 @@ def main():
     src = hip.DeviceBuffer(1 << 30)
     dst = hip.DeviceBuffer(1 << 30)
     src.fill_byte(1)
-    for _ in range(args.reps):
+    for _ in range(args.warmup + args.reps):
         hip.check(hip._load().hipMemcpy(hip.ctypes.c_void_p(dst.ptr), hip.ctypes.c_void_p(src.ptr), 1 << 30, 3),
                   "hipMemcpy D2D")
     hip.synchronize()
@@ -45,7 +48,7 @@ def main():
     buf = hip.DeviceBuffer(n * 8)
     g = mrg8_amd.mrg8_cuda(workloads.SEED)
     for fmt in (mrg8_amd.F64, mrg8_amd.U32):
-        for _ in range(args.reps):
+        for _ in range(args.warmup + args.reps):
             g.generate(buf, n, fmt)
         g.synchronize()
     buf.free()
@@ -55,9 +58,10 @@ def main():
         out = hip.DeviceBuffer(total * 8)
         starts = np.concatenate([[0], np.cumsum(lengths)[:-1]]).astype(np.uint64)
         b = mrg8_amd.Batch(max_streams=len(lengths))
+        b.set_config(256, 1, 4096)  # the shape the autotuner picks for this batch on MI355X
         b.set([(states[i], 0, int(lengths[i]), out.ptr + 8 * int(starts[i]), mrg8_amd.F64)
                for i in range(len(lengths))])
-        for _ in range(args.reps):
+        for _ in range(args.warmup + args.reps):
             b.run()
         hip.synchronize()
         out.free()
@@ -68,7 +72,7 @@ def main():
         j = mrg8_amd.Batch(max_streams=m, max_tiles=m)
         j.set_config(256, 8, 256)
         j.set([(s0, int(steps[i]), 256, out.ptr + 1024 * i, mrg8_amd.U32) for i in range(m)])
-        for _ in range(args.reps):
+        for _ in range(args.warmup + args.reps):
             j.run()
         hip.synchronize()
         out.free()
