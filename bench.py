@@ -126,6 +126,29 @@ def bench_single(args):
     return out
 
 
+def bench_host_form(n=1 << 26, reps=3):
+    """mrg8_rand into pageable host memory (device generation + D2H copy):
+    the PCIe-inclusive rate a host-buffer caller sees; never the headline."""
+    import ctypes
+
+    import mrg8_amd
+    from mrg8_amd import capi
+
+    g = mrg8_amd.mrg8_cuda(SEED)
+    out = np.empty(n, dtype=np.float64)
+    out.fill(0.0)  # touch the pages before timing
+    times = []
+    for _ in range(reps + 1):
+        t0 = time.perf_counter()
+        capi.call("mrg8_rand", g._h, ctypes.c_void_p(out.ctypes.data), ctypes.c_uint64(n))
+        times.append(time.perf_counter() - t0)
+    g.close()
+    mean = sum(times[1:]) / reps
+    log(f"host form (mrg8_rand, pageable, PCIe-inclusive): {n / mean / 1e9:.2f} G samples/s")
+    return {"outputs": n, "ms": mean * 1e3, "gsps": n / mean / 1e9, "gbs": n * 8 / mean / 1e9,
+            "destination": "pageable host memory (numpy)"}
+
+
 def bench_batched(args):
     import mrg8_amd
     from mrg8_amd import hip, workloads
@@ -309,6 +332,7 @@ def main():
         line["batched"] = bench_batched(args)
         line["jumps"] = bench_jumps(args)
         line["served"] = bench_served(args, line["batched"]["gsps"])
+        line["host_form_f64"] = bench_host_form()
     if not args.quick and not args.no_comparators:
         from mrg8_amd import comparators
 

@@ -2,19 +2,25 @@
 //
 // Replaces the CUDA generation kernels rng_outer_32 and rng_inner
 // (cuda/mrg8_cuda.cu:201-399).  Design:
-//   * A wave owns a contiguous run of outputs of one stream.  The 8-word
-//     state is wave-uniform (scalar registers); lane k holds the centred
-//     coefficients of row 0 of A^(k+1) (one record per output it makes), so
-//     one iteration yields 64 * PL consecutive outputs with eight signed
-//     32x32->64 multiply-adds each (v_mad_i64_i32) and a Mersenne fold.
+//   * A wave owns a contiguous run (tile) of outputs of one stream.  The
+//     8-word state is wave-uniform (scalar registers); lane k holds the
+//     centred coefficients of row 0 of A^(PL k + i + 1), i < PL, so one
+//     iteration yields 64 * PL consecutive outputs with eight signed
+//     32x32->64 multiply-adds each (v_mad_i64_i32, the PL dot products
+//     interleaved) and a Mersenne fold.
 //   * The next state is the last eight outputs, read back with v_readlane;
 //     no LDS and no barrier in the loop.
-//   * Outputs leave through coalesced non-temporal vector stores (8 or 16
-//     bytes per lane).
-//   * A wave reaches its first position with a jump: the state is multiplied
-//     by A^(2^b) for every set bit b of the 64-bit offset.  Eight lanes per
-//     matrix compute one row each; eight matrices are fetched per round so a
-//     round costs one memory latency.
+//   * Outputs leave through coalesced non-temporal vector stores (16 bytes
+//     per lane once the destination is 16-byte aligned).
+//   * A wave reaches its first position with a jump: each non-zero
+//     hexadecimal digit d at position k of the 64-bit offset is one product
+//     with A^(d 16^k) (at most 16 steps).  Eight lane groups of eight lanes
+//     jump eight states at once, lane r of a group computing row r, and the
+//     next digit's record is fetched one step ahead.
+//   * k_fill: one stream, one tile per wave; the block's start states are
+//     jumped by its first waves and shared through LDS.
+//   * k_batch: many streams (segments), a persistent grid striding over
+//     tiles, tables and counts read from one device block.
 #include "mrg8_device.hpp"
 
 namespace mrg8 {
