@@ -267,6 +267,23 @@ def cpu_baselines(args):
                                "avx512": oracle.vec_has_avx512(),
                                "sample": "knl/mrg8_vec.h outer-product restatement, 2^28 (x-1)/p doubles, "
                                          "10 runs, first dropped, mean of 9"}
+    # the 4096-stream batch, OpenMP over streams (each stream generated serially
+    # in cache-sized blocks and checksummed, not stored)
+    from mrg8_amd import workloads
+
+    states, lengths = workloads.batched()
+    zeros = np.zeros(len(lengths), dtype=np.uint64)
+    fmts = np.zeros(len(lengths), dtype=np.int32)
+    t0 = time.perf_counter()
+    oracle.stream_wsums(states, zeros, lengths, fmts)
+    wall = time.perf_counter() - t0
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
+    res["cpu_baseline_batched"] = {"value": float(lengths.sum()) / wall / 1e9, "unit": "G samples/s",
+                                   "cores": threads, "kind": "port",
+                                   "sample": "all 4096 streams of the batched workload (2.13e9 float64), "
+                                             "oracle serial recurrence per stream, OpenMP over streams, "
+                                             "outputs checksummed in 4096-value blocks instead of stored; one run"}
+    log(f"cpu batched: {res['cpu_baseline_batched']['value']:.2f} G samples/s on {threads} threads")
     try:
         res["host_cpu"] = subprocess.run(["sh", "-c", "grep -m1 'model name' /proc/cpuinfo | cut -d: -f2"],
                                          capture_output=True, text=True).stdout.strip()

@@ -97,6 +97,30 @@ def test_launch_shapes_identical(gpu, oracle_mod, block, tile):
     assert np.array_equal(got.view(np.uint64), want.view(np.uint64))
 
 
+def test_beyond_2e32_outputs(gpu, oracle_mod):
+    """One call of 2^32 + 4096 words (16 GiB): 64-bit counts and offsets
+    throughout (the reference takes an int n and jumps 32 bits on the
+    device).  Windows around 2^31 and 2^32 and the final state are checked
+    against the oracle's jump."""
+    from mrg8_amd.hip import DeviceBuffer
+
+    n = (1 << 32) + 4096
+    say("beyond-2^32", n)
+    buf = DeviceBuffer(n * 4)
+    try:
+        g = gpu.mrg8_cuda(SEED)
+        g.generate(buf.ptr, n, 2)
+        g.synchronize()
+        s0 = oracle_mod.seed_state(SEED)
+        for start, count in ((0, 4096), ((1 << 31) - 2048, 4096), ((1 << 32) - 2048, 6144)):
+            got = buf.copy_to_host(np.uint32, count, start * 4)
+            want, _ = oracle_mod.generate(oracle_mod.jump(s0, start), count, 2)
+            assert np.array_equal(got, want), start
+        assert np.array_equal(g.get_state(), oracle_mod.jump(s0, n))
+    finally:
+        buf.free()
+
+
 def test_consecutive_calls_continue(gpu, oracle_mod):
     say("consecutive")
     g = gpu.mrg8_cuda(SEED)
