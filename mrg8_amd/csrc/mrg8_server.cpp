@@ -131,7 +131,8 @@ struct mrg8_server {
   Arena arena;
   std::unordered_map<uint64_t, std::unique_ptr<Request>> reqs;
   std::deque<Request*> queued, active;
-  std::vector<Request*> order;  // scratch: service order of one step
+  std::vector<Request*> order;                       // scratch: service order of one step
+  std::vector<std::pair<Request*, uint64_t>> taken;  // scratch: chunks of the step being built
   std::deque<Request*> done_unreported;
   std::deque<Step*> inflight;  // launch order
   std::vector<std::unique_ptr<Step>> steps;
@@ -206,11 +207,14 @@ int launch_steps(mrg8_server* s) {
                                      [&](const Request* r) { return now - r->t_admit > age_limit; });
     std::stable_sort(mid, order.end(),
                      [](const Request* a, const Request* b) { return a->count - a->issued < b->count - b->issued; });
+    std::vector<std::pair<Request*, uint64_t>>& taken = s->taken;
+    taken.clear();
     for (auto it = order.begin(); it != order.end() && budget > 0 && n < cap; ++it) {
       Request* r = *it;
       const uint64_t left = r->count - r->issued;
       uint64_t take = std::min(left, std::min(budget, per_req));
       if (take < left && take > s->launch.tile) take = take / s->launch.tile * s->launch.tile;
+      taken.emplace_back(r, take);
       BatchSegment& sg = segs[n++];
       std::memset(sg.state, 0, sizeof sg.state);
       sg.stream = r->stream;
@@ -230,7 +234,18 @@ int launch_steps(mrg8_server* s) {
                     s->active.end());
     if (n == 0) return MRG8_OK;
     step->slot.finalise(n, s->launch.tile);
-    if ((rc = step->slot.run(s->launch, step->stream, s->copy_stream)) != MRG8_OK) return rc;
+    if ((rc = step->slot.run(s->launch, step->stream, s->copy_stream)) != MRG8_OK) {
+      // nothing of this step was launched: its chunks go back to their requests
+      for (auto& [r, take] : taken) {
+        if (r->st == RState::Issued) {
+          r->st = RState::Active;
+          s->active.push_back(r);
+        }
+        r->issued -= take;
+      }
+      step->finishing.clear();
+      return rc;
+    }
     step->busy = true;
     s->inflight.push_back(step);
     s->counters.steps += 1;
