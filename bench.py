@@ -204,8 +204,7 @@ def bench_served(args, capacity_gsps):
     states, arrivals, ids, sizes = workloads.served(n_requests=n_req, rate_per_s=rate)
     srv = mrg8_amd.Server(num_streams=len(states), max_output_bytes=64 << 30, max_inflight=4096,
                           step_outputs=args.step_outputs)
-    for i in range(len(states)):
-        srv.set_stream(i, states[i], 0)
+    srv.set_streams(states)
     # warm-up trace (separate streams' positions advance; states stay resident)
     srv.replay(arrivals[:200] / 10, ids[:200], sizes[:200], fmt=mrg8_amd.F64)
     c0 = srv.counters()

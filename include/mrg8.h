@@ -239,6 +239,10 @@ typedef struct mrg8_completion {
 int mrg8_server_create(int device, const mrg8_server_config* cfg, uint32_t base_seed, mrg8_server_t* out);
 int mrg8_server_destroy(mrg8_server_t s);
 int mrg8_server_set_stream(mrg8_server_t s, uint32_t stream_id, const uint32_t state[8], uint64_t position);
+/* Streams first .. first+count-1 at once: states[8 i .. 8 i + 7] and
+ * positions[i] (NULL: position 0); one upload. */
+int mrg8_server_set_streams(mrg8_server_t s, uint32_t first, uint32_t count, const uint32_t* states,
+                            const uint64_t* positions);
 int mrg8_server_get_stream(mrg8_server_t s, uint32_t stream_id, uint32_t state[8], uint64_t* position);
 /* Queues a request for the next `count` outputs of stream `stream_id`; the
  * request is given the stream's next `count` positions at submit time, so

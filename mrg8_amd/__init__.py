@@ -279,6 +279,17 @@ class Server:
         capi.call("mrg8_server_set_stream", self._h, ctypes.c_uint32(stream_id), _u32x8(state),
                   ctypes.c_uint64(position))
 
+    def set_streams(self, states, positions=None, first: int = 0) -> None:
+        """Sets streams first .. first+len(states)-1 in one upload."""
+        st = np.ascontiguousarray(np.asarray(states, dtype=np.uint32).reshape(-1, 8))
+        pos = None
+        if positions is not None:
+            pos = np.ascontiguousarray(np.asarray(positions, dtype=np.uint64))
+            assert pos.size == st.shape[0]
+        capi.call("mrg8_server_set_streams", self._h, ctypes.c_uint32(first), ctypes.c_uint32(st.shape[0]),
+                  st.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+                  ctypes.c_void_p(pos.ctypes.data) if pos is not None else None)
+
     def get_stream(self, stream_id: int):
         st = (ctypes.c_uint32 * 8)()
         pos = ctypes.c_uint64()

@@ -110,6 +110,7 @@ SIGNATURES = {
                                           ctypes.POINTER(H)]),
     "mrg8_server_destroy": (ctypes.c_int, [H]),
     "mrg8_server_set_stream": (ctypes.c_int, [H, ctypes.c_uint32, u32p, ctypes.c_uint64]),
+    "mrg8_server_set_streams": (ctypes.c_int, [H, ctypes.c_uint32, ctypes.c_uint32, u32p, ctypes.c_void_p]),
     "mrg8_server_get_stream": (ctypes.c_int, [H, ctypes.c_uint32, u32p, ctypes.POINTER(ctypes.c_uint64)]),
     "mrg8_server_submit": (ctypes.c_int, [H, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int,
                                           ctypes.POINTER(ctypes.c_uint64)]),

@@ -16,10 +16,11 @@
 // than `age_limit_us` ago first (admission order), then the rest by fewest
 // outputs left, so small requests do not queue behind large ones and large
 // ones cannot starve.  Up to `steps_in_flight` steps are in flight, each on
-// its own stream so that a step starts while the previous one's last waves
-// drain; a request arriving while they run joins the next one.  Each step's
-// staging copy runs on a separate copy stream, overlapping the launches
-// before it.
+// its own stream, so that a step's staging copy and launch overlap the
+// steps before it and a step starts while the previous one's last waves
+// drain; a request arriving while they run joins the next one.  Steps launch
+// the kernel directly: every step has a new block, and a plain launch costs
+// the host less than a graph replay.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -122,7 +123,6 @@ struct mrg8_server {
   DeviceContext* ctx = nullptr;
   mrg8_server_config cfg{};
   LaunchConfig launch;
-  hipStream_t copy_stream = nullptr;  // step staging copies, overlapping the launches before
   uint32_t* d_states = nullptr;
   std::vector<uint32_t> h_states;
   std::vector<uint64_t> cursor;
@@ -234,7 +234,7 @@ int launch_steps(mrg8_server* s) {
                     s->active.end());
     if (n == 0) return MRG8_OK;
     step->slot.finalise(n, s->launch.tile);
-    if ((rc = step->slot.run(s->launch, step->stream, s->copy_stream)) != MRG8_OK) {
+    if ((rc = step->slot.run(s->launch, step->stream, /*direct=*/true)) != MRG8_OK) {
       // nothing of this step was launched: its chunks go back to their requests
       for (auto& [r, take] : taken) {
         if (r->st == RState::Issued) {
@@ -304,8 +304,7 @@ extern "C" int mrg8_server_create(int device, const mrg8_server_config* cfg, uin
   s->cursor.assign(nst, 0);
   s->pending_per_stream.assign(nst, 0);
   for (size_t i = 0; i < nst; ++i) seed_state(base_seed + (uint32_t)i, &s->h_states[i * 8]);
-  hipError_t e = hipStreamCreateWithFlags(&s->copy_stream, hipStreamNonBlocking);
-  if (e == hipSuccess) e = hipMalloc((void**)&s->d_states, nst * 8 * sizeof(uint32_t));
+  hipError_t e = hipMalloc((void**)&s->d_states, nst * 8 * sizeof(uint32_t));
   if (e == hipSuccess)
     e = hipMemcpy(s->d_states, s->h_states.data(), nst * 8 * sizeof(uint32_t), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMalloc((void**)&s->d_arena, s->cfg.max_output_bytes);
@@ -343,7 +342,6 @@ extern "C" int mrg8_server_destroy(mrg8_server_t s) {
   DeviceGuard dg(s->ctx ? s->ctx->device : 0);
   for (auto& st : s->steps)
     if (st->stream) (void)hipStreamSynchronize(st->stream);
-  if (s->copy_stream) (void)hipStreamSynchronize(s->copy_stream);
   for (auto& st : s->steps) {
     st->slot.release();  // waits for its last run
     if (st->stream) (void)hipStreamDestroy(st->stream);
@@ -351,7 +349,6 @@ extern "C" int mrg8_server_destroy(mrg8_server_t s) {
   s->steps.clear();
   if (s->d_arena) (void)hipFree(s->d_arena);
   if (s->d_states) (void)hipFree(s->d_states);
-  if (s->copy_stream) (void)hipStreamDestroy(s->copy_stream);
   delete s;
   return MRG8_OK;
 }
@@ -372,6 +369,32 @@ extern "C" int mrg8_server_set_stream(mrg8_server_t s, uint32_t stream_id, const
     if (e == hipSuccess) e = hipStreamSynchronize(st->stream);
   if (e == hipSuccess)
     e = hipMemcpy(s->d_states + (size_t)stream_id * 8, state, 8 * sizeof(uint32_t), hipMemcpyHostToDevice);
+  if (e != hipSuccess) return hip_fail(e, "state upload");
+  return MRG8_OK;
+}
+
+extern "C" int mrg8_server_set_streams(mrg8_server_t s, uint32_t first, uint32_t count, const uint32_t* states,
+                                       const uint64_t* positions) {
+  clear_error();
+  if (!s || (count && !states)) return fail(MRG8_ERR_ARG, "null argument");
+  if (first > s->cfg.num_streams || count > s->cfg.num_streams - first)
+    return fail(MRG8_ERR_NOTFOUND, "streams [%u, %u) out of range", first, first + count);
+  for (uint32_t i = 0; i < count; ++i) {
+    if (!state_valid(states + 8ull * i)) return fail(MRG8_ERR_ARG, "stream %u: state words must be at most 2^31 - 1", first + i);
+    if (s->pending_per_stream[first + i])
+      return fail(MRG8_ERR_STATE, "stream %u has unreleased requests", first + i);
+  }
+  if (count == 0) return MRG8_OK;
+  DeviceGuard dg(s->ctx->device);
+  std::memcpy(&s->h_states[(size_t)first * 8], states, (size_t)count * 8 * sizeof(uint32_t));
+  for (uint32_t i = 0; i < count; ++i) s->cursor[first + i] = positions ? positions[i] : 0;
+  // in-flight steps of other streams read the table: copy once they finish
+  hipError_t e = hipSuccess;
+  for (auto& st : s->steps)
+    if (e == hipSuccess) e = hipStreamSynchronize(st->stream);
+  if (e == hipSuccess)
+    e = hipMemcpy(s->d_states + (size_t)first * 8, states, (size_t)count * 8 * sizeof(uint32_t),
+                  hipMemcpyHostToDevice);
   if (e != hipSuccess) return hip_fail(e, "state upload");
   return MRG8_OK;
 }

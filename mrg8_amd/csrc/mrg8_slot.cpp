@@ -90,6 +90,16 @@ uint64_t BatchSlot::finalise(uint32_t count, uint64_t tile) {
   return t;
 }
 
+BatchArgs BatchSlot::args(const LaunchConfig& cfg) const {
+  BatchArgs a{};
+  a.block = dmem_;
+  a.tile = cfg.tile;
+  a.states = states_;
+  a.lane = ctx_->tables.lane;
+  a.jump = ctx_->tables.jump;
+  return a;
+}
+
 int BatchSlot::capture(const LaunchConfig& cfg) {
   if (have_graph_ && cfg.block_threads == captured_.block_threads && cfg.blocks_per_cu == captured_.blocks_per_cu &&
       cfg.tile == captured_.tile)
@@ -103,12 +113,7 @@ int BatchSlot::capture(const LaunchConfig& cfg) {
   hipStream_t cs = nullptr;
   hipError_t e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
   if (e != hipSuccess) return hip_fail(e, "hipStreamCreate(capture)");
-  BatchArgs a{};
-  a.block = dmem_;
-  a.tile = cfg.tile;
-  a.states = states_;
-  a.lane = ctx_->tables.lane;
-  a.jump = ctx_->tables.jump;
+  const BatchArgs a = args(cfg);
   const int grid = ctx_->num_cu * cfg.blocks_per_cu;
   e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
   if (e == hipSuccess) {
@@ -124,28 +129,30 @@ int BatchSlot::capture(const LaunchConfig& cfg) {
   return MRG8_OK;
 }
 
-int BatchSlot::run(const LaunchConfig& cfg, hipStream_t stream, hipStream_t copy_stream) {
+int BatchSlot::run(const LaunchConfig& cfg, hipStream_t stream, bool direct) {
   if (count_ == 0 || ntiles_ == 0) return MRG8_OK;
   if (ntiles_ > max_tiles_) return fail(MRG8_ERR_CAPACITY, "staged work needs more than max_tiles tiles");
   DeviceGuard dg(ctx_->device);
-  int rc = capture(cfg);
+  int rc = direct ? MRG8_OK : capture(cfg);
   if (rc) return rc;
   hipError_t e = hipSuccess;
   if (dirty_) {  // replays of an unchanged batch copy nothing
     // The device block may be overwritten once this slot's previous launch
-    // has finished: order the copy after it (a no-op on the launch stream).
-    hipStream_t cs = copy_stream ? copy_stream : stream;
-    if (run_pending_ && cs != stream) e = hipStreamWaitEvent(cs, run_done_, 0);
-    if (e == hipSuccess) e = hipMemcpyAsync(dmem_, hmem_, used_bytes_, hipMemcpyHostToDevice, cs);
-    if (e == hipSuccess) e = hipEventRecord(staged_, cs);
-    if (e == hipSuccess && copy_stream) e = hipStreamWaitEvent(stream, staged_, 0);
+    // has finished: on the same stream the copy is ordered after it; on
+    // another stream it waits for it.
+    if (run_pending_) e = hipStreamWaitEvent(stream, run_done_, 0);
+    if (e == hipSuccess) e = hipMemcpyAsync(dmem_, hmem_, used_bytes_, hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess) e = hipEventRecord(staged_, stream);
     if (e != hipSuccess) return hip_fail(e, "batch staging copy");
     copy_pending_ = true;
     dirty_ = false;
   }
-  e = hipGraphLaunch(exec_, stream);
+  if (direct)
+    e = launch_batch(args(cfg), ctx_->num_cu * cfg.blocks_per_cu, cfg.block_threads, stream);
+  else
+    e = hipGraphLaunch(exec_, stream);
   if (e == hipSuccess) e = hipEventRecord(run_done_, stream);
-  if (e != hipSuccess) return hip_fail(e, "hipGraphLaunch");
+  if (e != hipSuccess) return hip_fail(e, direct ? "batch launch" : "hipGraphLaunch");
   run_pending_ = true;
   return MRG8_OK;
 }

@@ -4,10 +4,10 @@
 // segments, tile prefix sums, tile -> segment map), sized for `cap` segments
 // and `max_tiles` tiles; a pinned host block of the same size stages it.  A
 // run copies the used prefix host-to-device in one copy when it changed since
-// the last run (optionally on a separate copy stream, so that it overlaps
-// the previous launch) and replays a captured graph holding the persistent
-// kernel; the kernel reads the counts and table offsets from the header, so
-// the graph is captured once per launch configuration.
+// the last run, on the launch stream, and replays a captured graph holding
+// the persistent kernel (or launches the kernel directly); the kernel reads
+// the counts and table offsets from the header, so the graph is captured once
+// per launch configuration.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -49,9 +49,10 @@ class BatchSlot {
   uint64_t staged_tiles() const { return ntiles_; }
   uint32_t staged_count() const { return count_; }
 
-  // Copies the staged block (on `copy_stream` when given, else on `stream`)
-  // and replays the launch on `stream`.
-  int run(const LaunchConfig& cfg, hipStream_t stream, hipStream_t copy_stream = nullptr);
+  // Copies the staged block (when it changed) and launches on `stream`: by
+  // replaying the captured graph, or with `direct` by a plain kernel launch
+  // (cheaper on the host for a launch whose block changes every time).
+  int run(const LaunchConfig& cfg, hipStream_t stream, bool direct = false);
   // The last run finished (non-blocking query).
   bool done();
   int wait_done();
@@ -59,6 +60,7 @@ class BatchSlot {
 
  private:
   int capture(const LaunchConfig& cfg);
+  BatchArgs args(const LaunchConfig& cfg) const;
 
   DeviceContext* ctx_ = nullptr;
   uint32_t cap_ = 0;

@@ -279,6 +279,36 @@ def test_server_requests_match_oracle(gpu, oracle_mod):
     assert st["completed"] == 300 and st["bytes_inflight"] == 0 and st["queued"] == 0
 
 
+def test_server_set_streams_bulk(gpu, oracle_mod):
+    """Streams set in one upload (with start positions) serve from those
+    states and positions; the others keep their seeded states."""
+    rng = np.random.default_rng(41)
+    states = rng.integers(1, 2**31 - 1, size=(20, 8)).astype(np.uint32)
+    positions = [int(x) for x in rng.integers(0, 2**62, size=20, dtype=np.uint64)]
+    srv = gpu.Server(num_streams=32, max_output_bytes=16 << 20, base_seed=500, step_outputs=1 << 18)
+    srv.set_streams(states, positions, first=4)
+    for sid in (3, 4, 13, 23, 24):
+        st, pos = srv.get_stream(sid)
+        if 4 <= sid < 24:
+            assert list(st) == list(states[sid - 4]) and pos == positions[sid - 4]
+        else:
+            assert list(st) == list(oracle_mod.seed_state(500 + sid)) and pos == 0
+    reqs = {}
+    for sid in (3, 4, 13, 23, 24):
+        reqs[srv.submit(sid, 3001, 0)] = sid
+    say("server-set-streams")
+
+    def check(c):
+        sid = reqs[c["request_id"]]
+        base = states[sid - 4] if 4 <= sid < 24 else oracle_mod.seed_state(500 + sid)
+        ok, _ = check_request(gpu, oracle_mod, c, base, 0)
+        assert ok, sid
+
+    drain(srv, len(reqs), check)
+    with pytest.raises(Exception):
+        srv.set_streams(states, first=20)  # past the last stream
+
+
 def test_server_alone_equals_mixed(gpu, oracle_mod):
     """A request gives the same outputs alone and in the middle of a mix."""
     seq = [(5, 1000, 0), (5, 5000, 2), (5, 77777, 0)]  # the third is the probe request
