@@ -218,7 +218,7 @@ typedef struct mrg8_server_config {
   uint64_t tile_outputs;       /* outputs per work item (0 = automatic) */
   int32_t block_threads;       /* 0 = automatic */
   int32_t blocks_per_cu;       /* 0 = automatic */
-  uint32_t steps_in_flight;    /* launches in flight at once, 1..8 (0 = 2) */
+  uint32_t steps_in_flight;    /* launches in flight at once, 1..8 (0 = 3) */
   uint32_t age_limit_us;       /* requests admitted longer ago than this are served first,
                                   in admission order; the rest shortest-remaining first (0 = 1000) */
 } mrg8_server_config;

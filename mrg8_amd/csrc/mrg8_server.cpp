@@ -278,17 +278,16 @@ extern "C" int mrg8_server_create(int device, const mrg8_server_config* cfg, uin
   s->ctx = ctx;
   s->cfg = *cfg;
   if (!s->cfg.max_inflight) s->cfg.max_inflight = 4096;
-  // Defaults measured on MI355X with the log-uniform request mix at 490-600
+  // Defaults measured on MI355X with the log-uniform request mix at 490-680
   // G samples/s offered: 4096-output tiles, one 256-thread block per CU (the
-  // batched kernel's best shape), 2^27-output steps (32 tiles per wave),
-  // two steps in flight, 1 ms age limit.  Smaller steps cut the median but
-  // stop keeping up above ~550 G samples/s; more steps in flight only add
-  // queueing in front of a new request.
+  // batched kernel's best shape), 2^26-output steps (16 tiles per wave),
+  // three steps in flight, 1 ms age limit.  2^25-output steps stop keeping up
+  // above ~600 G samples/s; 2^27-output steps add ~0.2-0.4 ms to p99.
   if (!s->cfg.tile_outputs) s->cfg.tile_outputs = 4096;
-  if (!s->cfg.step_outputs) s->cfg.step_outputs = 1ull << 27;
+  if (!s->cfg.step_outputs) s->cfg.step_outputs = 1ull << 26;
   if (!s->cfg.block_threads) s->cfg.block_threads = 256;
   if (!s->cfg.blocks_per_cu) s->cfg.blocks_per_cu = 1;
-  if (!s->cfg.steps_in_flight) s->cfg.steps_in_flight = 2;
+  if (!s->cfg.steps_in_flight) s->cfg.steps_in_flight = 3;
   if (!s->cfg.age_limit_us) s->cfg.age_limit_us = 1000;
   if (s->cfg.steps_in_flight > kMaxSteps) return fail(MRG8_ERR_ARG, "steps_in_flight must be at most %u", kMaxSteps);
   if (s->cfg.tile_outputs % 256 || s->cfg.block_threads % 64 || s->cfg.block_threads > 1024 ||
