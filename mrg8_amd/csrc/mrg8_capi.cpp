@@ -74,7 +74,7 @@ int device_context(int device, DeviceContext** out) {
   if ((e = hipSetDevice(device)) != hipSuccess) return hip_fail(e, "hipSetDevice");
 
   // Coefficient records: block rows (row 0 of A^(k+1)) and jump rows.
-  std::vector<CoefRecord> lane(kLaneRows), jump((size_t)kJumpDigits * 15 * 8);
+  std::vector<CoefRecord> lane(kLaneRows), jump((size_t)kJumpPositions * kJumpValues * 8);
   const std::vector<uint32_t> rows = block_rows(kLaneRows);
   for (int k = 0; k < kLaneRows; ++k) {
     if (!all_nonzero(&rows[(size_t)k * 8])) {
@@ -83,16 +83,17 @@ int device_context(int device, DeviceContext** out) {
     }
     lane[k] = make_record(&rows[(size_t)k * 8]);
   }
-  // Radix-16 jump table: A^(d 16^k) for digit d = 1..15 at position k, built
-  // from A^(16^k) = A^(2^(4k)) by repeated products.  Rows of the small
-  // powers (A^1..A^7) have structural zeros; the support mask keeps their
-  // exact-zero results exact.
+  // Jump table: A^(d 2^(kJumpBits k)) for digit d = 1..kJumpValues at
+  // position k, built from A^(2^(kJumpBits k)) (a table entry) by repeated
+  // products.  Rows of the small powers (A^1..A^7) have structural zeros;
+  // the support mask keeps their exact-zero results exact.
   const JumpTable& jt = host_jump_table();
-  for (int k = 0; k < kJumpDigits; ++k) {
-    std::vector<uint32_t> m(jt.at(4 * k), jt.at(4 * k) + 64), p = m;
-    for (int d = 1; d <= 15; ++d) {
+  for (int k = 0; k < kJumpPositions; ++k) {
+    std::vector<uint32_t> m(jt.at(kJumpBits * k), jt.at(kJumpBits * k) + 64), p = m;
+    for (int d = 1; d <= kJumpValues; ++d) {
       if (d > 1) p = mat_mul(p.data(), m.data());
-      for (int r = 0; r < 8; ++r) jump[((size_t)k * 15 + (d - 1)) * 8 + r] = make_record(&p[(size_t)r * 8]);
+      for (int r = 0; r < 8; ++r)
+        jump[((size_t)k * kJumpValues + (d - 1)) * 8 + r] = make_record(&p[(size_t)r * 8]);
     }
   }
 

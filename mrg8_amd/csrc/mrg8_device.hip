@@ -12,11 +12,11 @@
 //     no LDS and no barrier in the loop.
 //   * Outputs leave through coalesced non-temporal vector stores (16 bytes
 //     per lane once the destination is 16-byte aligned).
-//   * A wave reaches its first position with a jump: each non-zero
-//     hexadecimal digit d at position k of the 64-bit offset is one product
-//     with A^(d 16^k) (at most 16 steps).  Eight lane groups of eight lanes
-//     jump eight states at once, lane r of a group computing row r, and the
-//     next digit's record is fetched one step ahead.
+//   * A wave reaches its first position with a jump: each non-zero byte d
+//     at position k of the 64-bit offset is one product with A^(d 256^k)
+//     (at most 8 steps).  Eight lane groups of eight lanes jump eight states
+//     at once, lane r of a group computing row r, and the next digit's
+//     record is fetched one step ahead.
 //   * k_fill: one stream, one tile per wave; the block's start states are
 //     jumped by its first waves and shared through LDS.
 //   * k_batch: many streams (segments), a persistent grid striding over
@@ -94,18 +94,19 @@ __device__ __forceinline__ void centre(const uint32_t s[8], int32_t e[8]) {
 
 // Advances eight states at once: lane group g = lane / 8 holds its own state
 // (replicated in the group's lanes, s[] per lane) and its own 64-bit offset;
-// for the group's lowest non-zero hexadecimal digit d at position k, lane r
-// of the group computes row r of A^(d 16^k), and the group's eight new words
-// are gathered back with ds_bpermute.  Groups with no digits left idle until
-// every group is done; a 64-bit jump takes at most 16 steps.
+// for the group's lowest non-zero digit d (kJumpBits bits) at position k,
+// lane r of the group computes row r of A^(d 2^(kJumpBits k)), and the
+// group's eight new words are gathered back with ds_bpermute.  Groups with no
+// digits left idle until every group is done; a 64-bit jump takes at most
+// kJumpPositions steps.
 __device__ __forceinline__ const CoefRecord* digit_row(const CoefRecord* __restrict__ jrec, uint64_t off, int row) {
-  const int k = __builtin_ctzll(off) >> 2;
-  const int d = (int)((off >> (4 * k)) & 15u);
-  return jrec + ((k * 15 + (d - 1)) * 8 + row);
+  const int k = __builtin_ctzll(off) / kJumpBits;
+  const int d = (int)((off >> (kJumpBits * k)) & (uint64_t)kJumpValues);
+  return jrec + ((k * kJumpValues + (d - 1)) * 8 + row);
 }
 
 __device__ __forceinline__ uint64_t drop_digit(uint64_t off) {
-  return off & ~(15ull << (__builtin_ctzll(off) & ~3));
+  return off & ~((uint64_t)kJumpValues << (__builtin_ctzll(off) / kJumpBits * kJumpBits));
 }
 
 __device__ __forceinline__ void jump_groups(uint32_t s[8], uint64_t off, const CoefRecord* __restrict__ jrec) {

@@ -10,12 +10,18 @@
 namespace mrg8 {
 
 constexpr int kLaneRows = 256;   // block rows kept on the device: up to 4 outputs per lane
-constexpr int kJumpDigits = 16;  // device jumps take 64-bit offsets as 16 hexadecimal digits
+// Device jumps take a 64-bit offset as kJumpPositions digits of kJumpBits
+// bits; each non-zero digit is one product with a table matrix.
+constexpr int kJumpBits = 8;
+constexpr int kJumpPositions = 64 / kJumpBits;
+constexpr int kJumpValues = (1 << kJumpBits) - 1;  // non-zero digit values
 
 // Per-device constant tables.
 struct DeviceTables {
   CoefRecord* lane = nullptr;  // kLaneRows records: row 0 of A^(k+1), support 0xff
-  CoefRecord* jump = nullptr;  // kJumpDigits * 15 * 8 records: row r of A^(d 16^k) at ((k*15)+(d-1))*8+r
+  // kJumpPositions * kJumpValues * 8 records: row r of A^(d 2^(kJumpBits k))
+  // at ((k * kJumpValues) + (d - 1)) * 8 + r
+  CoefRecord* jump = nullptr;
 };
 
 // Wave-uniform starting point of a launch.
