@@ -78,6 +78,21 @@ def traffic_from_profiles(name):
     return None
 
 
+def valu_busy_from_profiles(name, kernel):
+    """VALUBusy (%) of `kernel` in the committed counter summary, if recorded."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        parts = line.split()
+        if len(parts) >= 3 and parts[0] == kernel and parts[1] == "VALUBusy":
+            try:
+                return float(parts[2])
+            except ValueError:
+                return None
+    return None
+
+
 def bench_single(args):
     import mrg8_amd
     from mrg8_amd import hip
@@ -334,6 +349,7 @@ def main():
                      "frac": round(achieved / PEAK_GBS, 4),
                      "traffic": traffic,
                      "kernel": "k_fill<F64,2>", "kernel_ms": round(f64["kernel_ms"], 4),
+                     "valu_busy_pct": valu_busy_from_profiles("fill_f64_2e28_counters.txt", "k_fill<F64,2>"),
                      "algorithmic_bytes_per_launch": f64["bytes"]},
         "u32": {"gsps": round(N_SINGLE * args.steps / single["u32"]["wall_s"] / 1e9, 2),
                 "kernel_ms": round(single["u32"]["kernel_ms"], 4),

@@ -258,3 +258,66 @@ def test_fill_autotune_keeps_outputs(gpu, oracle_mod, tmp_path):
         want, _ = oracle_mod.generate(oracle_mod.seed_state(SEED), n, fmt)
         assert np.array_equal(got.view(np.uint8), want.view(np.uint8))
     assert len(open(cache).read().splitlines()) == 2
+
+
+def test_threads_with_own_handles_and_streams(gpu, oracle_mod):
+    """Eight host threads, each with its own generator and HIP stream, fill
+    concurrently; every result matches the oracle."""
+    import threading
+
+    from mrg8_amd import hip
+
+    n = 300_007
+    results, errors = {}, []
+
+    def work(t):
+        try:
+            s = hip.Stream()
+            buf = hip.DeviceBuffer(n * 8)
+            g = gpu.mrg8_cuda(1000 + t)
+            g.jump_ahead(t * 12345)
+            for _ in range(3):  # the third call's outputs are kept
+                g.generate(buf, n, t % 3, stream=s)
+            s.synchronize()
+            results[t] = buf.copy_to_host(NP_DTYPE[t % 3], n)
+            buf.free()
+            s.destroy()
+        except Exception as e:  # pragma: no cover - reported below
+            errors.append(repr(e))
+
+    say("threads")
+    ths = [threading.Thread(target=work, args=(t,)) for t in range(8)]
+    for th in ths:
+        th.start()
+    for th in ths:
+        th.join()
+    assert not errors, errors
+    for t in range(8):
+        start = oracle_mod.jump(oracle_mod.seed_state(1000 + t), t * 12345 + 2 * n)
+        want, _ = oracle_mod.generate(start, n, t % 3)
+        assert np.array_equal(results[t].view(np.uint8), want.view(np.uint8)), t
+
+
+def test_generate_inside_a_hip_graph(gpu, oracle_mod):
+    """A generate call captured into a HIP graph (torch.cuda.graph) replays
+    the positions it had at capture: each replay rewrites the same outputs,
+    identical to the eager call; the handle's state advanced once, at
+    capture."""
+    import torch
+
+    n = 1 << 20
+    g = gpu.mrg8_cuda(SEED)
+    out = torch.zeros(n, dtype=torch.float64, device="cuda:0")
+    side = torch.cuda.Stream()
+    graph = torch.cuda.CUDAGraph()
+    say("graph-capture")
+    with torch.cuda.graph(graph, stream=side):
+        g.generate(out, n, 0, stream=side)
+    want, st = oracle_mod.generate(oracle_mod.seed_state(SEED), n, 0)
+    assert np.array_equal(g.get_state(), st)
+    for rep in range(3):
+        out.zero_()
+        torch.cuda.synchronize()
+        graph.replay()
+        torch.cuda.synchronize()
+        assert np.array_equal(out.cpu().numpy().view(np.uint64), want.view(np.uint64)), rep
