@@ -348,8 +348,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_GBS, 4),
                      "traffic": traffic,
-                     "kernel": "k_fill<F64,2>", "kernel_ms": round(f64["kernel_ms"], 4),
-                     "valu_busy_pct": valu_busy_from_profiles("fill_f64_2e28_counters.txt", "k_fill<F64,2>"),
+                     "kernel": "k_fill<F64,4>", "kernel_ms": round(f64["kernel_ms"], 4),
+                     "valu_busy_pct": valu_busy_from_profiles("fill_f64_2e28_counters.txt", "k_fill<F64,4>"),
                      "algorithmic_bytes_per_launch": f64["bytes"]},
         "u32": {"gsps": round(N_SINGLE * args.steps / single["u32"]["wall_s"] / 1e9, 2),
                 "kernel_ms": round(single["u32"]["kernel_ms"], 4),

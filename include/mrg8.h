@@ -112,8 +112,8 @@ int mrg8_advance_state_bits(const uint32_t state[8], const short bits[200], uint
 int mrg8_generate(mrg8_gen_t g, void* dptr, uint64_t n, int format, void* stream);
 /* Same, with an explicit launch shape: threads per block (64..1024, multiple
  * of 64), outputs per wave (rounded up to 64 * outputs per lane) and outputs
- * per lane (1, 2, or 4 for words; 1 or 2 for doubles); 0 means automatic for
- * each.  The output is identical for every shape. */
+ * per lane (1, 2 or 4); 0 means automatic for each.  The output is
+ * identical for every shape. */
 int mrg8_generate_shape(mrg8_gen_t g, void* dptr, uint64_t n, int format, void* stream,
                         int block_threads, uint64_t outputs_per_wave, int outputs_per_lane);
 /* Launch shape of a single-stream fill. */

@@ -136,23 +136,26 @@ int format_size(int format) {
   }
 }
 
-// Outputs per lane for a destination address: 16-byte stores when aligned.
+// Outputs per lane for a destination address: 16-byte stores when aligned
+// (words: one 16-byte store per lane; doubles: two, each part of a
+// contiguous 1 KiB store of the wave).
 int pick_per_lane(int format, const void* p) {
   const uintptr_t a = reinterpret_cast<uintptr_t>(p);
   if (format == MRG8_U32) return (a & 15u) == 0 ? 4 : ((a & 7u) == 0 ? 2 : 1);
-  return (a & 15u) == 0 ? 2 : 1;
+  return (a & 15u) == 0 ? 4 : 1;
 }
 
 // Default launch shape for a single-stream fill of n outputs.  Measured on
-// MI355X (2^28 outputs): doubles are fastest with 1024-thread blocks and 2048
-// outputs per wave, words with 256-thread blocks and 8192 per wave; small
-// fills shrink the per-wave run so that every SIMD still gets waves.
+// MI355X (2^28 outputs): doubles are fastest with 1024-thread blocks, 4
+// outputs per lane and 2048 outputs per wave, words with 256-thread blocks
+// and 8192 per wave; small fills shrink the per-wave run (down to four
+// iterations) so that every SIMD still gets waves.
 void default_shape(const DeviceContext& ctx, uint64_t n, int format, int per_lane, int* block, uint64_t* tile) {
   const uint64_t quantum = 64ull * per_lane;
   *block = format == MRG8_U32 ? 256 : 1024;
   uint64_t t = format == MRG8_U32 ? 8192 : 2048;
   const uint64_t min_waves = (uint64_t)ctx.num_cu * 4 * 4;  // four waves per SIMD
-  while (t > 16 * quantum && (n + t - 1) / t < min_waves) t /= 2;
+  while (t > 4 * quantum && (n + t - 1) / t < min_waves) t /= 2;
   *tile = std::max<uint64_t>(quantum, t / quantum * quantum);
 }
 
@@ -169,14 +172,13 @@ int fill_device(DeviceContext& ctx, const uint32_t state[8], uint64_t offset, vo
   int pl = pick_per_lane(format, main_ptr);
   if (per_lane) {
     if (per_lane != 1 && per_lane != 2 && per_lane != 4) return fail(MRG8_ERR_ARG, "outputs_per_lane must be 1, 2 or 4");
-    if (format != MRG8_U32 && per_lane == 4) return fail(MRG8_ERR_ARG, "doubles take at most 2 outputs per lane");
     pl = per_lane;
   }
   int def_block = 0;
   uint64_t def_tile = 0;
   const int cls = main_n ? 63 - __builtin_clzll(main_n) : 0;
   const TunedShape& tuned = ctx.tuned[format][cls];
-  if (tuned.valid && !per_lane && (tuned.per_lane == 1 || pick_per_lane(format, main_ptr) >= tuned.per_lane)) {
+  if (tuned.valid && !per_lane && (tuned.per_lane == 1 || (reinterpret_cast<uintptr_t>(main_ptr) & 15u) == 0)) {
     pl = tuned.per_lane;
     def_block = tuned.block_threads;
     def_tile = tuned.tile;
@@ -609,7 +611,6 @@ extern "C" int mrg8_tune_fill(int device, int format, uint64_t n, const char* ca
     const uint64_t tiles[] = {2048, 4096, 8192, 16384};
     const int pls[] = {1, 2, 4};
     for (int pl : pls) {
-      if (pl == 4 && format != MRG8_U32) continue;
       for (int bt : blocks)
         for (uint64_t tile : tiles) {
           if (tile < 64ull * pl * 8) continue;

@@ -160,30 +160,52 @@ __device__ __forceinline__ typename Out<FMT>::T convert_zero() {
   else return 0u;
 }
 
-// Non-temporal store of PL consecutive outputs of one lane.
+// Which outputs of an iteration (64 * PL consecutive outputs) a lane makes.
+// Doubles with PL = 4 use two halves: lane k makes outputs 2k, 2k+1 and
+// 128+2k, 128+2k+1, so each of the lane's two 16-byte stores belongs to a
+// fully contiguous 1 KiB store of the wave.  Otherwise lane k makes outputs
+// PL k .. PL k + PL - 1 (one store of 4, 8 or 16 bytes).
 template <int FMT, int PL>
-__device__ __forceinline__ void store_lane(typename Out<FMT>::T* p, const uint32_t r[PL]) {
+struct LaneLayout {
+  static constexpr bool kSplit = FMT != 2 && PL == 4;
+  static __device__ __forceinline__ uint32_t index(uint32_t lane, int i) {
+    if constexpr (kSplit) return 128u * (uint32_t)(i / 2) + 2u * lane + (uint32_t)(i % 2);
+    else return PL * lane + (uint32_t)i;
+  }
+  // the slot (0..PL-1) and lane that make output o of an iteration
+  static constexpr int slot_of(int o) { return kSplit ? 2 * (o / 128) + o % 2 : o % PL; }
+  static constexpr int lane_of(int o) { return kSplit ? (o % 128) / 2 : o / PL; }
+};
+
+// Non-temporal stores of one lane's PL outputs of the iteration at `base`.
+template <int FMT, int PL>
+__device__ __forceinline__ void store_lane(typename Out<FMT>::T* base, uint32_t lane, const uint32_t r[PL]) {
   if constexpr (FMT == 2 && PL == 4) {
     u4_t v = {r[0] + 1u, r[1] + 1u, r[2] + 1u, r[3] + 1u};
-    __builtin_nontemporal_store(v, reinterpret_cast<u4_t*>(p));
+    __builtin_nontemporal_store(v, reinterpret_cast<u4_t*>(base + 4 * lane));
   } else if constexpr (FMT == 2 && PL == 2) {
     u2_t v = {r[0] + 1u, r[1] + 1u};
-    __builtin_nontemporal_store(v, reinterpret_cast<u2_t*>(p));
+    __builtin_nontemporal_store(v, reinterpret_cast<u2_t*>(base + 2 * lane));
+  } else if constexpr (FMT != 2 && PL == 4) {
+    d2_t v0 = {convert<FMT>(r[0]), convert<FMT>(r[1])};
+    d2_t v1 = {convert<FMT>(r[2]), convert<FMT>(r[3])};
+    __builtin_nontemporal_store(v0, reinterpret_cast<d2_t*>(base + 2 * lane));
+    __builtin_nontemporal_store(v1, reinterpret_cast<d2_t*>(base + 128 + 2 * lane));
   } else if constexpr (FMT != 2 && PL == 2) {
     d2_t v = {convert<FMT>(r[0]), convert<FMT>(r[1])};
-    __builtin_nontemporal_store(v, reinterpret_cast<d2_t*>(p));
+    __builtin_nontemporal_store(v, reinterpret_cast<d2_t*>(base + 2 * lane));
   } else {
 #pragma unroll
-    for (int i = 0; i < PL; ++i) __builtin_nontemporal_store(convert<FMT>(r[i]), p + i);
+    for (int i = 0; i < PL; ++i) __builtin_nontemporal_store(convert<FMT>(r[i]), base + PL * lane + i);
   }
 }
 
-// This lane's PL records: row 0 of A^(PL lane + i + 1), i < PL.
-template <int PL>
+// This lane's PL records: row 0 of A^(o + 1) for each output o it makes.
+template <int FMT, int PL>
 __device__ __forceinline__ void load_lane_coefs(Coef (&k)[PL], const CoefRecord* __restrict__ lane_rec) {
-  const int lane = threadIdx.x & 63;
+  const uint32_t lane = threadIdx.x & 63;
 #pragma unroll
-  for (int i = 0; i < PL; ++i) k[i] = load_coef(lane_rec + PL * lane + i);
+  for (int i = 0; i < PL; ++i) k[i] = load_coef(lane_rec + LaneLayout<FMT, PL>::index(lane, i));
 }
 
 // Generates `cnt` outputs of the stream whose wave-uniform state (at the
@@ -192,9 +214,10 @@ __device__ __forceinline__ void load_lane_coefs(Coef (&k)[PL], const CoefRecord*
 template <int FMT, int PL>
 __device__ __forceinline__ void generate_run(typename Out<FMT>::T* __restrict__ dst, uint64_t cnt, uint32_t s[8],
                                              const Coef (&k)[PL]) {
+  typedef LaneLayout<FMT, PL> L;
   constexpr int kPer = 64 * PL;
-  const int lane = threadIdx.x & 63;
-  typename Out<FMT>::T* p = dst + PL * lane;
+  const uint32_t lane = threadIdx.x & 63;
+  typename Out<FMT>::T* base = dst;
   const uint64_t full = cnt / kPer;
   const uint64_t rem = cnt - full * kPer;
   uint32_t any = 0;
@@ -209,19 +232,22 @@ __device__ __forceinline__ void generate_run(typename Out<FMT>::T* __restrict__ 
     centre(s, e);
     uint32_t r[PL];
     dot_residues<PL>(k, e, r);
-    store_lane<FMT, PL>(p, r);
-    p += kPer;
+    store_lane<FMT, PL>(base, lane, r);
+    base += kPer;
     // next state: the last eight words, newest first
 #pragma unroll
-    for (int j = 0; j < 8; ++j) s[j] = __builtin_amdgcn_readlane(r[(kPer - 1 - j) % PL], (kPer - 1 - j) / PL) + 1u;
+    for (int j = 0; j < 8; ++j)
+      s[j] = __builtin_amdgcn_readlane(r[L::slot_of(kPer - 1 - j)], L::lane_of(kPer - 1 - j)) + 1u;
   }
   if (rem) {
     int32_t e[8];
     centre(s, e);
+    uint32_t r[PL];
+    dot_residues<PL>(k, e, r);
 #pragma unroll
     for (int i = 0; i < PL; ++i) {
-      const uint32_t r = dot_residue(k[i], e);
-      if ((uint64_t)(PL * lane + i) < rem) __builtin_nontemporal_store(convert<FMT>(r), p + i);
+      const uint32_t o = L::index(lane, i);
+      if (o < rem) __builtin_nontemporal_store(convert<FMT>(r[i]), base + o);
     }
   }
 }
@@ -299,7 +325,7 @@ __global__ __launch_bounds__(1024) void k_fill(FillArgs a) {
 #pragma unroll
   for (int j = 0; j < 8; ++j) s[j] = __builtin_amdgcn_readfirstlane(start_state[wib][j]);
   Coef k[PL];
-  load_lane_coefs<PL>(k, a.lane);
+  load_lane_coefs<FMT, PL>(k, a.lane);
   generate_run<FMT, PL>(static_cast<typename Out<FMT>::T*>(a.out) + first, cnt, s, k);
 }
 
@@ -354,7 +380,7 @@ __global__ __launch_bounds__(1024) void k_batch(BatchArgs a) {
   const uint64_t wave = (uint64_t)blockIdx.x * (blockDim.x >> 6) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   if (wave >= ntiles) return;
   Coef k[2];
-  load_lane_coefs<2>(k, a.lane);
+  load_lane_coefs<0, 2>(k, a.lane);  // the PL = 2 layout is the same for every format
   const uint64_t mine_total = (ntiles - wave + nwaves - 1) / nwaves;
   for (uint64_t j0 = 0; j0 < mine_total; j0 += kGroup) {
     const uint32_t ngrp = (uint32_t)min<uint64_t>(kGroup, mine_total - j0);
@@ -407,8 +433,10 @@ hipError_t launch_fill(const FillArgs& a, int format, int per_lane, int block_th
   switch (format * 8 + per_lane) {
     case 0 * 8 + 1: return fill_one<0, 1>(a, block_threads, stream);
     case 0 * 8 + 2: return fill_one<0, 2>(a, block_threads, stream);
+    case 0 * 8 + 4: return fill_one<0, 4>(a, block_threads, stream);
     case 1 * 8 + 1: return fill_one<1, 1>(a, block_threads, stream);
     case 1 * 8 + 2: return fill_one<1, 2>(a, block_threads, stream);
+    case 1 * 8 + 4: return fill_one<1, 4>(a, block_threads, stream);
     case 2 * 8 + 1: return fill_one<2, 1>(a, block_threads, stream);
     case 2 * 8 + 2: return fill_one<2, 2>(a, block_threads, stream);
     case 2 * 8 + 4: return fill_one<2, 4>(a, block_threads, stream);

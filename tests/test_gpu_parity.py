@@ -77,7 +77,7 @@ def test_tiny_unaligned_outputs(gpu, oracle_mod, n, pad):
         assert np.array_equal(g.get_state(), st)
 
 
-@pytest.mark.parametrize("fmt,per_lane", [(0, 1), (0, 2), (2, 1), (2, 2), (2, 4)])
+@pytest.mark.parametrize("fmt,per_lane", [(0, 1), (0, 2), (0, 4), (1, 4), (2, 1), (2, 2), (2, 4)])
 def test_outputs_per_lane(gpu, oracle_mod, fmt, per_lane):
     n = 777_777
     say("per-lane", fmt, per_lane)
@@ -85,6 +85,19 @@ def test_outputs_per_lane(gpu, oracle_mod, fmt, per_lane):
     got = run_device(gpu, g, n, fmt, pad_elems=1, block_threads=512, tile=8192, per_lane=per_lane)
     want, _ = oracle_mod.generate(oracle_mod.seed_state(SEED), n, fmt)
     assert np.array_equal(got.view(np.uint8), want.view(np.uint8))
+
+
+@pytest.mark.parametrize("n", [1, 100, 128, 129, 255, 256, 257, 511, 1000, 4095, 65537])
+@pytest.mark.parametrize("fmt,pad", [(0, 0), (0, 1), (1, 1)])
+def test_two_store_layout_ragged(gpu, oracle_mod, n, fmt, pad):
+    """Doubles with 4 outputs per lane (two 1 KiB stores per iteration):
+    partial last iterations and peeled starts."""
+    say("split-ragged", n, fmt, pad)
+    g = gpu.mrg8_cuda(SEED + n)
+    got = run_device(gpu, g, n, fmt, pad, block_threads=256, tile=512, per_lane=4)
+    want, st = oracle_mod.generate(oracle_mod.seed_state(SEED + n), n, fmt)
+    assert np.array_equal(got.view(np.uint8), want.view(np.uint8))
+    assert np.array_equal(g.get_state(), st)
 
 
 @pytest.mark.parametrize("block,tile", [(64, 128), (256, 1024), (512, 4096), (1024, 65536), (256, 1 << 20)])
