@@ -1,5 +1,8 @@
 """GPU parity: the gfx950 kernels against the oracle (plain-C restatement of
 original/mrg8.cpp), bit for bit, through the C ABI."""
+import json
+import os
+
 import numpy as np
 import pytest
 
@@ -8,6 +11,7 @@ from conftest import wsum
 pytestmark = pytest.mark.gpu
 
 SEED = 13579  # the reference's test seed (cuda/tests/test_mrg8_outer.cu:35)
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "ref_vectors.json")))
 NP_DTYPE = {0: np.float64, 1: np.float64, 2: np.uint32}
 
 
@@ -252,6 +256,10 @@ def test_c2_sampled_and_wsum(gpu, oracle_mod):
             assert got[int(i)].tobytes() == v[0].tobytes(), int(i)
         h, st = oracle_mod.hash_stream(s0, n, fmt)
         assert oracle_mod.fnv1a(got) == h
+        # the compiled reference's own hash and final state for this sequence
+        key = "randint_2^28" if fmt == 2 else "rand_2^28"
+        assert f"{oracle_mod.fnv1a(got):016x}" == GOLD["hash"][key]["fnv1a64"]
+        assert g.get_state().tolist() == GOLD["hash"][key]["state"]
         assert np.array_equal(g.get_state(), st)
         del got
 
@@ -334,3 +342,30 @@ def test_generate_inside_a_hip_graph(gpu, oracle_mod):
         graph.replay()
         torch.cuda.synchronize()
         assert np.array_equal(out.cpu().numpy().view(np.uint64), want.view(np.uint64)), rep
+
+
+@pytest.mark.parametrize("fmt", [0, 1])
+def test_c2_statistics_like_check_rand(gpu, fmt):
+    """The reference's only check of its generators is statistical
+    (common/rng_test.h:29-47: mean and standard deviation of the outputs).
+    2^28 doubles: mean 1/2 and standard deviation 1/sqrt(12) within about
+    4 standard errors, and every value inside its interval ((0, 1] for
+    x / p, [0, 1) for (x - 1) / p)."""
+    import torch
+
+    n = 1 << 28
+    say("check-rand", fmt)
+    t = torch.empty(n, dtype=torch.float64, device="cuda:0")
+    g = gpu.mrg8_cuda(SEED)
+    g.generate(t, n, fmt, stream=torch.cuda.current_stream())
+    mean = t.mean().item()
+    std = t.std().item()
+    lo, hi = t.min().item(), t.max().item()
+    del t
+    se = (1 / 12) ** 0.5 / n ** 0.5
+    assert abs(mean - 0.5) < 4 * se, mean
+    assert abs(std - (1 / 12) ** 0.5) < 4 * se, std  # the sample sd of U(0,1) varies by about 0.45 se
+    if fmt == 0:
+        assert lo > 0.0 and hi <= 1.0
+    else:
+        assert lo >= 0.0 and hi < 1.0
