@@ -4,12 +4,14 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
 #include <memory>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "mrg8_device.hpp"
@@ -220,9 +222,30 @@ struct mrg8_gen {
   uint32_t seed = 0;
   hipStream_t last_stream = nullptr;
   bool issued = false;
-  void* staging = nullptr;  // device buffer for the host-memory forms
-  size_t staging_bytes = 0;
+  // host-memory forms: two device chunks, two pinned host chunks, a stream
+  // and one event per chunk (allocated on first use)
+  void* dev_chunk[2] = {nullptr, nullptr};
+  void* pin_chunk[2] = {nullptr, nullptr};
+  hipEvent_t chunk_ready[2] = {nullptr, nullptr};
+  hipStream_t host_stream = nullptr;
 };
+
+namespace {
+
+void release_host_staging(mrg8_gen* g) {
+  if (g->host_stream) (void)hipStreamSynchronize(g->host_stream);
+  for (int i = 0; i < 2; ++i) {
+    if (g->dev_chunk[i]) (void)hipFree(g->dev_chunk[i]);
+    if (g->pin_chunk[i]) (void)hipHostFree(g->pin_chunk[i]);
+    if (g->chunk_ready[i]) (void)hipEventDestroy(g->chunk_ready[i]);
+    g->dev_chunk[i] = g->pin_chunk[i] = nullptr;
+    g->chunk_ready[i] = nullptr;
+  }
+  if (g->host_stream) (void)hipStreamDestroy(g->host_stream);
+  g->host_stream = nullptr;
+}
+
+}  // namespace
 
 // ---- library ----------------------------------------------------------------
 
@@ -279,7 +302,10 @@ extern "C" int mrg8_destroy(mrg8_gen_t g) {
     (void)hipSetDevice(g->ctx->device);
     (void)hipStreamSynchronize(g->last_stream);
   }
-  if (g->staging) (void)hipFree(g->staging);
+  if (g->host_stream) {
+    (void)hipSetDevice(g->ctx->device);
+    release_host_staging(g);
+  }
   delete g;
   return MRG8_OK;
 }
@@ -449,39 +475,80 @@ extern "C" int mrg8_state_cpy_DtH(mrg8_gen_t g) {
 
 // ---- host-memory forms ------------------------------------------------------------
 
+// Copies `bytes` from `src` to `dst` with up to 8 host threads (the pinned
+// chunk to the caller's pageable memory).
+static void parallel_copy(void* dst, const void* src, size_t bytes) {
+  const size_t min_part = (size_t)4 << 20;
+  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+  const size_t parts = std::min<size_t>(std::min(8u, hw), std::max<size_t>(1, bytes / min_part));
+  if (parts <= 1) {
+    std::memcpy(dst, src, bytes);
+    return;
+  }
+  const size_t per = (bytes / parts + 63) / 64 * 64;
+  std::vector<std::thread> th;
+  for (size_t i = 1; i < parts; ++i) {
+    const size_t off = i * per;
+    if (off >= bytes) break;
+    th.emplace_back([=] {
+      std::memcpy(static_cast<char*>(dst) + off, static_cast<const char*>(src) + off, std::min(per, bytes - off));
+    });
+  }
+  std::memcpy(dst, src, std::min(per, bytes));
+  for (auto& t : th) t.join();
+}
+
+// Host-memory forms: generate into device chunks and bring them back through
+// pinned chunks, two of each in flight: while the host copies chunk i out of
+// its pinned buffer, the device generates and transfers chunk i + 1.
 static int host_fill(mrg8_gen_t g, void* host, uint64_t n, int format) {
   if (!g) return fail(MRG8_ERR_ARG, "null generator");
   if (n == 0) return MRG8_OK;
   if (!host) return fail(MRG8_ERR_ARG, "null output pointer");
   const size_t esz = (size_t)format_size(format);
-  const size_t chunk_bytes = (size_t)64 << 20;
-  int prev = 0;
-  (void)hipGetDevice(&prev);
-  (void)hipSetDevice(g->ctx->device);
+  const size_t chunk_bytes = (size_t)32 << 20;
+  DeviceGuard dg(g->ctx->device);
   hipError_t e = hipSuccess;
-  if (!g->staging) {
-    e = hipMalloc(&g->staging, chunk_bytes);
-    if (e != hipSuccess) {
-      g->staging = nullptr;
-      (void)hipSetDevice(prev);
-      return hip_fail(e, "hipMalloc(staging)");
+  if (!g->host_stream) {
+    e = hipStreamCreateWithFlags(&g->host_stream, hipStreamNonBlocking);
+    for (int i = 0; i < 2 && e == hipSuccess; ++i) {
+      e = hipMalloc(&g->dev_chunk[i], chunk_bytes);
+      if (e == hipSuccess) e = hipHostMalloc(&g->pin_chunk[i], chunk_bytes, hipHostMallocDefault);
+      if (e == hipSuccess) e = hipEventCreateWithFlags(&g->chunk_ready[i], hipEventDisableTiming);
     }
-    g->staging_bytes = chunk_bytes;
+    if (e != hipSuccess) {
+      release_host_staging(g);
+      return hip_fail(e, "host staging allocation");
+    }
   }
   // Finish earlier stream-ordered work on this generator first.
   if (g->issued) (void)hipStreamSynchronize(g->last_stream);
   const uint64_t per = chunk_bytes / esz;
-  int rc = MRG8_OK;
-  for (uint64_t done = 0; done < n && rc == MRG8_OK;) {
-    const uint64_t m = std::min<uint64_t>(per, n - done);
-    rc = fill_device(*g->ctx, g->state, 0, g->staging, m, format, nullptr, 0, 0, 0);
-    if (rc) break;
+  const uint64_t nchunks = (n + per - 1) / per;
+  auto count = [&](uint64_t i) { return std::min<uint64_t>(per, n - i * per); };
+  auto issue = [&](uint64_t i) -> int {
+    const int b = (int)(i & 1);
+    const uint64_t m = count(i);
+    int rc = fill_device(*g->ctx, g->state, 0, g->dev_chunk[b], m, format, g->host_stream, 0, 0, 0);
+    if (rc) return rc;
     jump64(host_jump_table(), m, g->state);
-    e = hipMemcpy(static_cast<char*>(host) + done * esz, g->staging, m * esz, hipMemcpyDeviceToHost);
-    if (e != hipSuccess) rc = hip_fail(e, "hipMemcpy(D2H)");
-    done += m;
+    hipError_t x = hipMemcpyAsync(g->pin_chunk[b], g->dev_chunk[b], m * esz, hipMemcpyDeviceToHost, g->host_stream);
+    if (x == hipSuccess) x = hipEventRecord(g->chunk_ready[b], g->host_stream);
+    return x == hipSuccess ? MRG8_OK : hip_fail(x, "hipMemcpyAsync(D2H)");
+  };
+  int rc = issue(0);
+  if (rc == MRG8_OK && nchunks > 1) rc = issue(1);
+  for (uint64_t i = 0; i < nchunks && rc == MRG8_OK; ++i) {
+    const int b = (int)(i & 1);
+    e = hipEventSynchronize(g->chunk_ready[b]);
+    if (e != hipSuccess) {
+      rc = hip_fail(e, "hipEventSynchronize(D2H)");
+      break;
+    }
+    parallel_copy(static_cast<char*>(host) + i * per * esz, g->pin_chunk[b], count(i) * esz);
+    if (i + 2 < nchunks) rc = issue(i + 2);
   }
-  (void)hipSetDevice(prev);
+  if (rc) (void)hipStreamSynchronize(g->host_stream);  // leave nothing in flight on an error
   return rc;
 }
 

@@ -42,6 +42,21 @@ int pick_per_lane(int format, const void* p);
 int fill_device(DeviceContext& ctx, const uint32_t state[8], uint64_t offset, void* dptr, uint64_t n, int format,
                 hipStream_t stream, int block_threads, uint64_t tile, int per_lane);
 
+// RAII device switch.
+class DeviceGuard {
+ public:
+  explicit DeviceGuard(int dev) : dev_(dev) {
+    (void)hipGetDevice(&prev_);
+    if (prev_ != dev_) (void)hipSetDevice(dev_);
+  }
+  ~DeviceGuard() {
+    if (prev_ != dev_) (void)hipSetDevice(prev_);
+  }
+
+ private:
+  int prev_ = 0, dev_ = 0;
+};
+
 // Host wall clock in seconds (steady).
 double now_seconds();
 

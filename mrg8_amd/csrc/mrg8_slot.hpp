@@ -81,19 +81,4 @@ class BatchSlot {
   bool have_graph_ = false;
 };
 
-// RAII device switch.
-class DeviceGuard {
- public:
-  explicit DeviceGuard(int dev) : dev_(dev) {
-    (void)hipGetDevice(&prev_);
-    if (prev_ != dev_) (void)hipSetDevice(dev_);
-  }
-  ~DeviceGuard() {
-    if (prev_ != dev_) (void)hipSetDevice(prev_);
-  }
-
- private:
-  int prev_ = 0, dev_ = 0;
-};
-
 }  // namespace mrg8

@@ -233,6 +233,29 @@ def test_host_forms(gpu, oracle_mod):
     assert e == float(want[6001]) * rn
 
 
+def test_host_forms_many_chunks(gpu, oracle_mod):
+    """Host-memory forms larger than the staging chunks (32 MiB): 5 chunks of
+    doubles and 2.5 of words, plus a device call in between; the sequence
+    continues across all of them."""
+    say("host-forms-chunks")
+    g = gpu.mrg8_cuda(SEED + 1)
+    nd = 5 * (4 << 20) + 12345   # 4 Mi doubles per chunk
+    nw = (8 << 20) * 2 + 777     # 8 Mi words per chunk
+    a = g.rand(nd)
+    b = g.randint(nw)
+    c = run_device(gpu, g, 1001, 0)
+    d = g.rand(3)
+    want, st = oracle_mod.generate(oracle_mod.seed_state(SEED + 1), nd, 0)
+    assert np.array_equal(a.view(np.uint64), want.view(np.uint64))
+    want, st = oracle_mod.generate(st, nw, 2)
+    assert np.array_equal(b, want)
+    want, st = oracle_mod.generate(st, 1001, 0)
+    assert np.array_equal(c.view(np.uint64), want.view(np.uint64))
+    want, st = oracle_mod.generate(st, 3, 0)
+    assert np.array_equal(d.view(np.uint64), want.view(np.uint64))
+    assert np.array_equal(g.get_state(), st)
+
+
 def test_c2_sampled_and_wsum(gpu, oracle_mod):
     """2^28 outputs in u32 and f64: 1e4 sampled indices against jump+1 output,
     and a position-weighted checksum of the whole sequence."""
