@@ -184,10 +184,12 @@ typedef struct mrg8_stream_desc {
 } mrg8_stream_desc;
 
 /* Launch configuration (picked by mrg8_batch_autotune or set by hand; a new
- * batch starts with 256 threads, 1 block per CU, 4096-output tiles). */
+ * batch starts with 1024 threads, one tile per wave, 2048-output tiles). */
 typedef struct mrg8_batch_config {
   int32_t block_threads;  /* 64..1024, multiple of 64 */
-  int32_t blocks_per_cu;  /* persistent grid = CUs * blocks_per_cu */
+  int32_t blocks_per_cu;  /* 0: one tile per wave (grid from the tile count; best for long
+                             segments); > 0: persistent grid of CUs * blocks_per_cu blocks
+                             (best for many short segments with long jumps) */
   uint64_t tile_outputs;  /* outputs per work item, multiple of 256 */
 } mrg8_batch_config;
 

@@ -338,8 +338,8 @@ __device__ __forceinline__ uint32_t peel_count(const T* p) {
 // One tile of a batch segment at dst: peels up to one double (three words)
 // to reach 16-byte alignment (only tile 0 of a segment can need it), then
 // runs the wide-store loop.
-template <int FMT>
-__device__ __forceinline__ void batch_tile(void* out, uint64_t cnt, uint32_t s[8], const Coef (&k)[2],
+template <int FMT, int PL>
+__device__ __forceinline__ void batch_tile(void* out, uint64_t cnt, uint32_t s[8], const Coef (&k)[PL],
                                            const CoefRecord* __restrict__ lane_rec) {
   typedef typename Out<FMT>::T T;
   T* dst = static_cast<T*>(out);
@@ -350,13 +350,56 @@ __device__ __forceinline__ void batch_tile(void* out, uint64_t cnt, uint32_t s[8
     dst += peel;
     cnt -= peel;
   }
-  if (cnt) generate_run<FMT, 2>(dst, cnt, s, k);
+  if (cnt) generate_run<FMT, PL>(dst, cnt, s, k);
 }
 
 __device__ __forceinline__ uint64_t readlane_u64(uint64_t v, int lane) {
   const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)v, lane);
   const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(v >> 32), lane);
   return (uint64_t)lo | ((uint64_t)hi << 32);
+}
+
+// The tables of a batch block (see BatchHeader).
+struct BatchTables {
+  const BatchSegment* segs;
+  const uint64_t* tile_begin;
+  const uint32_t* tile_seg;
+  uint64_t ntiles;
+};
+
+__device__ __forceinline__ BatchTables batch_tables(const uint8_t* block) {
+  const BatchHeader* hdr = reinterpret_cast<const BatchHeader*>(block);
+  BatchTables t;
+  t.segs = reinterpret_cast<const BatchSegment*>(block + sizeof(BatchHeader));
+  t.tile_begin = reinterpret_cast<const uint64_t*>(block + hdr->tile_begin_off);
+  t.tile_seg = reinterpret_cast<const uint32_t*>(block + hdr->tile_seg_off);
+  t.ntiles = hdr->ntiles;
+  return t;
+}
+
+// Destination, output count, stream position and state source of tile t.
+struct TileInfo {
+  uint64_t dst;
+  uint64_t cnt;
+  uint64_t offset;
+  const uint32_t* state;
+  int32_t fmt;
+};
+
+__device__ __forceinline__ TileInfo lookup_tile(const BatchTables& tab, const BatchArgs& a, uint64_t t) {
+  const uint32_t seg = tab.tile_seg[t];
+  const BatchSegment* sp = tab.segs + seg;
+  TileInfo ti;
+  ti.fmt = sp->format;
+  const uint64_t kt = t - tab.tile_begin[seg];  // tile number within the segment
+  const uint64_t head = segment_head(sp->out, ti.fmt, sp->count);
+  const uint64_t first = kt ? head + kt * a.tile : 0;
+  ti.cnt = min(sp->count, head + (kt + 1) * a.tile) - first;
+  ti.dst = reinterpret_cast<uint64_t>(sp->out) + first * (ti.fmt == 2 ? 4u : 8u);
+  ti.offset = sp->offset + first;
+  const uint32_t sid = sp->stream;
+  ti.state = sid == kInlineState ? sp->state : a.states + 8ull * sid;
+  return ti;
 }
 
 // Persistent batched kernel.  With W waves in the grid, wave w takes tiles
@@ -371,11 +414,8 @@ constexpr uint32_t kGroup = 8;
 __global__ __launch_bounds__(1024) void k_batch(BatchArgs a) {
   const int lane = threadIdx.x & 63;
   const int grp = lane >> 3;
-  const BatchHeader* hdr = reinterpret_cast<const BatchHeader*>(a.block);
-  const BatchSegment* segs = reinterpret_cast<const BatchSegment*>(a.block + sizeof(BatchHeader));
-  const uint64_t* tile_begin = reinterpret_cast<const uint64_t*>(a.block + hdr->tile_begin_off);
-  const uint32_t* tile_seg = reinterpret_cast<const uint32_t*>(a.block + hdr->tile_seg_off);
-  const uint64_t ntiles = hdr->ntiles;
+  const BatchTables tab = batch_tables(a.block);
+  const uint64_t ntiles = tab.ntiles;
   const uint64_t nwaves = (uint64_t)gridDim.x * (blockDim.x >> 6);
   const uint64_t wave = (uint64_t)blockIdx.x * (blockDim.x >> 6) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   if (wave >= ntiles) return;
@@ -385,33 +425,76 @@ __global__ __launch_bounds__(1024) void k_batch(BatchArgs a) {
   for (uint64_t j0 = 0; j0 < mine_total; j0 += kGroup) {
     const uint32_t ngrp = (uint32_t)min<uint64_t>(kGroup, mine_total - j0);
     const bool mine = (uint32_t)grp < ngrp;
-    const uint64_t t = (j0 + (mine ? grp : 0)) * nwaves + wave;
-    const uint32_t seg = tile_seg[t];
-    const BatchSegment* sp = segs + seg;
-    const int32_t fmt = sp->format;
-    const uint64_t kt = t - tile_begin[seg];  // tile number within the segment
-    const uint64_t head = segment_head(sp->out, fmt, sp->count);
-    const uint64_t first = kt ? head + kt * a.tile : 0;
-    const uint64_t cnt = min(sp->count, head + (kt + 1) * a.tile) - first;
-    const uint64_t dst = reinterpret_cast<uint64_t>(sp->out) + first * (fmt == 2 ? 4u : 8u);
-    const uint32_t sid = sp->stream;
-    const uint32_t* src = sid == kInlineState ? sp->state : a.states + 8ull * sid;
+    const TileInfo ti = lookup_tile(tab, a, (j0 + (mine ? grp : 0)) * nwaves + wave);
     uint32_t s[8];
 #pragma unroll
-    for (int j = 0; j < 8; ++j) s[j] = src[j];
-    jump_groups(s, mine ? sp->offset + first : 0, a.jump);
+    for (int j = 0; j < 8; ++j) s[j] = ti.state[j];
+    jump_groups(s, mine ? ti.offset : 0, a.jump);
     for (uint32_t g = 0; g < ngrp; ++g) {
       uint32_t u[8];
 #pragma unroll
       for (int j = 0; j < 8; ++j) u[j] = __builtin_amdgcn_readlane(s[j], 8 * g);
-      void* out = reinterpret_cast<void*>(readlane_u64(dst, 8 * g));
-      const uint64_t ucnt = readlane_u64(cnt, 8 * g);
-      switch (__builtin_amdgcn_readlane(fmt, 8 * g)) {
-        case 0: batch_tile<0>(out, ucnt, u, k, a.lane); break;
-        case 1: batch_tile<1>(out, ucnt, u, k, a.lane); break;
-        default: batch_tile<2>(out, ucnt, u, k, a.lane); break;
+      void* out = reinterpret_cast<void*>(readlane_u64(ti.dst, 8 * g));
+      const uint64_t ucnt = readlane_u64(ti.cnt, 8 * g);
+      switch (__builtin_amdgcn_readlane(ti.fmt, 8 * g)) {
+        case 0: batch_tile<0, 2>(out, ucnt, u, k, a.lane); break;
+        case 1: batch_tile<1, 2>(out, ucnt, u, k, a.lane); break;
+        default: batch_tile<2, 2>(out, ucnt, u, k, a.lane); break;
       }
     }
+  }
+}
+
+// One tile of a batch with the fill's layout (4 outputs per lane).
+template <int FMT>
+__device__ __forceinline__ void batch_tile_alone(void* out, uint64_t cnt, uint32_t s[8],
+                                                 const CoefRecord* __restrict__ lane_rec) {
+  Coef k[4];
+  load_lane_coefs<FMT, 4>(k, lane_rec);
+  batch_tile<FMT, 4>(out, cnt, s, k, lane_rec);
+}
+
+// Batched kernel with one tile per wave (grid = tiles / waves per block), the
+// structure of k_fill: waves 0-1 of a block look up and jump the start states
+// of the block's tiles (8-lane groups) and leave them in LDS; every wave then
+// generates its own tile and exits.
+__global__ __launch_bounds__(1024) void k_batch_tiles(BatchArgs a) {
+  __shared__ uint32_t start_state[16][8];
+  __shared__ uint64_t start_dst[16], start_cnt[16];
+  __shared__ int32_t start_fmt[16];
+  const int lane = threadIdx.x & 63;
+  const BatchTables tab = batch_tables(a.block);
+  const uint32_t wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const uint32_t nw = blockDim.x >> 6;
+  const uint64_t tile0 = (uint64_t)blockIdx.x * nw;  // < ntiles: the grid covers the tiles exactly
+  if (wib * 8 < nw) {
+    const uint32_t target = wib * 8 + (lane >> 3);
+    const bool mine = target < nw && tile0 + target < tab.ntiles;
+    const TileInfo ti = lookup_tile(tab, a, mine ? tile0 + target : tile0);
+    uint32_t s[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) s[j] = ti.state[j];
+    jump_groups(s, mine ? ti.offset : 0, a.jump);
+    if (mine && (lane & 7) == 0) {
+#pragma unroll
+      for (int j = 0; j < 8; ++j) start_state[target][j] = s[j];
+      start_dst[target] = ti.dst;
+      start_cnt[target] = ti.cnt;
+      start_fmt[target] = ti.fmt;
+    }
+  }
+  __syncthreads();
+  if (tile0 + wib >= tab.ntiles) return;
+  uint32_t u[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) u[j] = __builtin_amdgcn_readfirstlane(start_state[wib][j]);
+  const uint64_t d = start_dst[wib], c = start_cnt[wib];
+  void* out = reinterpret_cast<void*>(readlane_u64(d, 0));
+  const uint64_t cnt = readlane_u64(c, 0);
+  switch (__builtin_amdgcn_readfirstlane(start_fmt[wib])) {
+    case 0: batch_tile_alone<0>(out, cnt, u, a.lane); break;
+    case 1: batch_tile_alone<1>(out, cnt, u, a.lane); break;
+    default: batch_tile_alone<2>(out, cnt, u, a.lane); break;
   }
 }
 
@@ -446,6 +529,15 @@ hipError_t launch_fill(const FillArgs& a, int format, int per_lane, int block_th
 
 hipError_t launch_batch(const BatchArgs& a, int grid, int block_threads, hipStream_t stream) {
   hipLaunchKernelGGL(k_batch, dim3(grid), dim3(block_threads), 0, stream, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_tiles(const BatchArgs& a, uint64_t ntiles, int block_threads, hipStream_t stream) {
+  const uint64_t wpb = (uint64_t)block_threads / 64;
+  const uint64_t blocks = (ntiles + wpb - 1) / wpb;
+  if (blocks == 0) return hipSuccess;
+  if (blocks > 0x7fffffffull) return hipErrorInvalidConfiguration;
+  hipLaunchKernelGGL(k_batch_tiles, dim3((uint32_t)blocks), dim3(block_threads), 0, stream, a);
   return hipGetLastError();
 }
 

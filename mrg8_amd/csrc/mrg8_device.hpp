@@ -107,5 +107,7 @@ struct BatchArgs {
 // Launchers (mrg8_device.hip).  All are asynchronous on `stream`.
 hipError_t launch_fill(const FillArgs& a, int format, int per_lane, int block_threads, hipStream_t stream);
 hipError_t launch_batch(const BatchArgs& a, int grid, int block_threads, hipStream_t stream);
+// One tile per wave: ceil(ntiles / waves per block) blocks.
+hipError_t launch_batch_tiles(const BatchArgs& a, uint64_t ntiles, int block_threads, hipStream_t stream);
 
 }  // namespace mrg8

@@ -100,9 +100,15 @@ BatchArgs BatchSlot::args(const LaunchConfig& cfg) const {
   return a;
 }
 
+hipError_t BatchSlot::launch(const LaunchConfig& cfg, hipStream_t stream) const {
+  if (cfg.blocks_per_cu == 0) return launch_batch_tiles(args(cfg), ntiles_, cfg.block_threads, stream);
+  return launch_batch(args(cfg), ctx_->num_cu * cfg.blocks_per_cu, cfg.block_threads, stream);
+}
+
 int BatchSlot::capture(const LaunchConfig& cfg) {
+  // a one-tile-per-wave grid depends on the tile count, a persistent one does not
   if (have_graph_ && cfg.block_threads == captured_.block_threads && cfg.blocks_per_cu == captured_.blocks_per_cu &&
-      cfg.tile == captured_.tile)
+      cfg.tile == captured_.tile && (cfg.blocks_per_cu > 0 || captured_ntiles_ == ntiles_))
     return MRG8_OK;
   if (run_pending_) (void)hipEventSynchronize(run_done_);
   if (exec_) (void)hipGraphExecDestroy(exec_);
@@ -113,11 +119,9 @@ int BatchSlot::capture(const LaunchConfig& cfg) {
   hipStream_t cs = nullptr;
   hipError_t e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
   if (e != hipSuccess) return hip_fail(e, "hipStreamCreate(capture)");
-  const BatchArgs a = args(cfg);
-  const int grid = ctx_->num_cu * cfg.blocks_per_cu;
   e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
   if (e == hipSuccess) {
-    hipError_t e1 = launch_batch(a, grid, cfg.block_threads, cs);
+    hipError_t e1 = launch(cfg, cs);
     hipError_t e2 = hipStreamEndCapture(cs, &graph_);
     e = e1 != hipSuccess ? e1 : e2;
   }
@@ -125,6 +129,7 @@ int BatchSlot::capture(const LaunchConfig& cfg) {
   (void)hipStreamDestroy(cs);
   if (e != hipSuccess) return hip_fail(e, "graph capture");
   captured_ = cfg;
+  captured_ntiles_ = ntiles_;
   have_graph_ = true;
   return MRG8_OK;
 }
@@ -147,10 +152,7 @@ int BatchSlot::run(const LaunchConfig& cfg, hipStream_t stream, bool direct) {
     copy_pending_ = true;
     dirty_ = false;
   }
-  if (direct)
-    e = launch_batch(args(cfg), ctx_->num_cu * cfg.blocks_per_cu, cfg.block_threads, stream);
-  else
-    e = hipGraphLaunch(exec_, stream);
+  e = direct ? launch(cfg, stream) : hipGraphLaunch(exec_, stream);
   if (e == hipSuccess) e = hipEventRecord(run_done_, stream);
   if (e != hipSuccess) return hip_fail(e, direct ? "batch launch" : "hipGraphLaunch");
   run_pending_ = true;

@@ -20,8 +20,8 @@ namespace mrg8 {
 
 struct LaunchConfig {
   int block_threads = 256;
-  int blocks_per_cu = 8;
-  uint64_t tile = 65536;
+  int blocks_per_cu = 1;  // 0: one tile per wave (k_batch_tiles); > 0: persistent grid (k_batch)
+  uint64_t tile = 4096;
 };
 
 class BatchSlot {
@@ -61,6 +61,7 @@ class BatchSlot {
  private:
   int capture(const LaunchConfig& cfg);
   BatchArgs args(const LaunchConfig& cfg) const;
+  hipError_t launch(const LaunchConfig& cfg, hipStream_t stream) const;
 
   DeviceContext* ctx_ = nullptr;
   uint32_t cap_ = 0;
@@ -78,6 +79,7 @@ class BatchSlot {
   hipGraph_t graph_ = nullptr;
   hipGraphExec_t exec_ = nullptr;
   LaunchConfig captured_{};
+  uint64_t captured_ntiles_ = 0;
   bool have_graph_ = false;
 };
 

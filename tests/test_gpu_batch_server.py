@@ -39,7 +39,9 @@ class Packed:
         return self.buf.copy_to_host(DT[self.fmts[i]], int(self.counts[i]), self.offsets[i])
 
 
-def test_batch_mixed_segments(gpu, oracle_mod):
+@pytest.mark.parametrize("shapes", [((256, 4, 4096), (512, 2, 16384)), ((1024, 0, 4096), (256, 0, 512))],
+                         ids=["persistent", "tile-per-wave"])
+def test_batch_mixed_segments(gpu, oracle_mod, shapes):
     rng = np.random.default_rng(21)
     n = 200
     states = rng.integers(0, 2**31, size=(n, 8)).astype(np.uint32)
@@ -51,7 +53,7 @@ def test_batch_mixed_segments(gpu, oracle_mod):
     offs[:20] = list(range(20))  # small offsets take the shift rows of the jump
     pk = Packed(counts, fmts, gap_elems=int(rng.integers(1, 4)))
     b = gpu.Batch(max_streams=n, max_tiles=1 << 20)
-    b.set_config(256, 4, 4096)
+    b.set_config(*shapes[0])
     segs = [(states[i], offs[i], int(counts[i]), pk.ptr(i), int(fmts[i])) for i in range(n)]
     b.set(segs)
     for rep in range(3):  # graph replays give identical results
@@ -64,10 +66,11 @@ def test_batch_mixed_segments(gpu, oracle_mod):
             want, _ = oracle_mod.generate(oracle_mod.jump(states[i], offs[i]), int(counts[i]), int(fmts[i]))
             got = pk.fetch(i)
             assert np.array_equal(got.view(np.uint8), want.view(np.uint8)), (rep, i)
-        b.set_config(512, 2, 16384)  # a new shape, same outputs
+        b.set_config(*shapes[1])  # a new shape, same outputs
 
 
-def test_batch_tile_heads(gpu, oracle_mod):
+@pytest.mark.parametrize("bpc", [1, 0], ids=["persistent", "tile-per-wave"])
+def test_batch_tile_heads(gpu, oracle_mod, bpc):
     """Segments at every element offset within a 128-byte line, with counts
     around the head (the outputs before the first 128-byte boundary, which
     tile 0 of a segment also writes) and the tile size: outputs match the
@@ -89,7 +92,7 @@ def test_batch_tile_heads(gpu, oracle_mod):
     buf = DeviceBuffer(slot * len(cases))
     buf.fill_byte(0xAB)
     b = gpu.Batch(max_streams=len(cases), max_tiles=1 << 16)
-    b.set_config(256, 1, tile)  # a small grid: every wave takes several tiles
+    b.set_config(256, bpc, tile)  # persistent: a small grid, every wave takes several tiles
     b.set([(states[i], offs[i], c, buf.ptr + i * slot + m, fmt) for i, (fmt, _, m, c) in enumerate(cases)])
     say("batch-tile-heads", len(cases))
     b.run()
@@ -175,7 +178,8 @@ def test_batch_errors(gpu):
         b.set_config(100, 1, 1024)
 
 
-def test_c3_batched_full_size(gpu, oracle_mod):
+@pytest.mark.parametrize("shape", [None, (1024, 0, 4096)], ids=["default", "tile-per-wave"])
+def test_c3_batched_full_size(gpu, oracle_mod, shape):
     """4096 streams of log-uniform length in [2^10, 2^22), float64, one launch;
     every stream checked by its position-weighted checksum."""
     from mrg8_amd import workloads
@@ -187,6 +191,8 @@ def test_c3_batched_full_size(gpu, oracle_mod):
     buf = DeviceBuffer(total * 8)
     starts = np.concatenate([[0], np.cumsum(lengths)[:-1]]).astype(np.uint64)
     b = gpu.Batch(max_streams=len(lengths))
+    if shape:
+        b.set_config(*shape)
     b.set([(states[i], 0, int(lengths[i]), buf.ptr + 8 * int(starts[i]), 0) for i in range(len(lengths))])
     b.run()
     gpu.hip.synchronize()
@@ -199,7 +205,8 @@ def test_c3_batched_full_size(gpu, oracle_mod):
     assert not bad, bad[:10]
 
 
-def test_c4_jump_heavy_full_size(gpu, oracle_mod):
+@pytest.mark.parametrize("shape", [(256, 8, 256), (1024, 0, 256)], ids=["persistent", "tile-per-wave"])
+def test_c4_jump_heavy_full_size(gpu, oracle_mod, shape):
     """65536 streams from one seed, each jumped by a uniform step in [0, 2^63),
     then 256 words: every stream against the oracle's jump + generation."""
     from mrg8_amd import workloads
@@ -210,9 +217,9 @@ def test_c4_jump_heavy_full_size(gpu, oracle_mod):
     s0 = gpu.seed_state(workloads.SEED)
     buf = DeviceBuffer(n * 1024)
     b = gpu.Batch(max_streams=n, max_tiles=n)
-    b.set_config(256, 8, 256)
+    b.set_config(*shape)
     b.set([(s0, int(steps[i]), 256, buf.ptr + 1024 * i, 2) for i in range(n)])
-    say("c4", n)
+    say("c4", n, shape)
     b.run()
     gpu.hip.synchronize()
     host = buf.copy_to_host(np.uint32, n * 256).reshape(n, 256)
