@@ -183,7 +183,8 @@ def bench_batched(args):
     gbs = total * 8 / (km * 1e-3) / 1e9
     log(f"batched: {total / (km * 1e-3) / 1e9:.1f} G samples/s, {gbs:.0f} GB/s, cfg {cfg}")
     return {"streams": len(lengths), "outputs": total, "ms_per_launch": km, "gsps": total / (km * 1e-3) / 1e9,
-            "gbs": gbs, "frac_of_peak": gbs / PEAK_GBS, "config": cfg, "launch": "persistent, hipGraph replay"}
+            "gbs": gbs, "frac_of_peak": gbs / PEAK_GBS, "config": cfg,
+            "launch": "hipGraph replay, " + ("one tile per wave" if cfg["blocks_per_cu"] == 0 else "persistent grid")}
 
 
 def bench_jumps(args):
