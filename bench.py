@@ -312,7 +312,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
-    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--quick", action="store_true", help="single-stream figures only")
     ap.add_argument("--served-requests", type=int, default=100_000)
     ap.add_argument("--served-load", type=float, default=0.7)
