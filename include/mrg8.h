@@ -75,6 +75,9 @@ int mrg8_init_device(int device);
  * generator is bound to the current HIP device.  The jump table is built in
  * memory from its closed form; no jump_matrix.txt is read. */
 int mrg8_create(uint32_t seed, mrg8_gen_t* out);
+/* Frees the handle.  Generation already issued completes on its streams:
+ * its kernels read only their arguments and tables that live until the
+ * process ends, so destroy does not wait for them. */
 int mrg8_destroy(mrg8_gen_t g);
 /* mrg8_cuda::seed_init (cuda/mrg8_cuda.cu:66-70). */
 int mrg8_seed_init(mrg8_gen_t g, uint32_t seed);
@@ -144,7 +147,9 @@ int mrg8_inner(mrg8_gen_t g, double* d_ran, int n, int TNUM);
  * state is always current here; this waits for the generator's outstanding
  * work and returns. */
 int mrg8_state_cpy_DtH(mrg8_gen_t g);
-/* Waits until all generation issued through `g` has finished. */
+/* Waits until all generation issued through `g` has finished: synchronises
+ * every stream `g` generated on since the last call (those streams must
+ * still exist). */
 int mrg8_synchronize(mrg8_gen_t g);
 
 /* ---- generator: host-memory forms (class mrg8) --------------------------- */

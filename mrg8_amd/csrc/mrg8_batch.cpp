@@ -1,7 +1,8 @@
-// Batched generation: many stream segments of uneven length in one
-// persistent launch, replayed from a captured HIP graph (see mrg8_slot.hpp).
-// Waves pull fixed-size tiles from a global counter until it passes the tile
-// count, so uneven segment lengths balance automatically.
+// Batched generation: many stream segments of uneven length in one launch,
+// replayed from a captured HIP graph (see mrg8_slot.hpp).  Segments are cut
+// into fixed-size tiles numbered by a prefix sum; either every wave takes one
+// tile, or a persistent grid of W waves takes tiles w, w + W, ..., so uneven
+// segment lengths balance without atomics.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -215,8 +216,12 @@ extern "C" int mrg8_batch_autotune(mrg8_batch_t b, void* stream, const char* cac
   DeviceGuard dg(b->ctx->device);
   hipStream_t s = static_cast<hipStream_t>(stream);
   hipEvent_t e0 = nullptr, e1 = nullptr;
-  (void)hipEventCreate(&e0);
-  (void)hipEventCreate(&e1);
+  hipError_t ce = hipEventCreate(&e0);
+  if (ce == hipSuccess) ce = hipEventCreate(&e1);
+  if (ce != hipSuccess) {
+    if (e0) (void)hipEventDestroy(e0);
+    return hip_fail(ce, "hipEventCreate");
+  }
   // threads-per-block x waves-per-CU x tile size, the counterpart of the
   // reference's manual TNUM x BS sweep (cuda/tests/test_mrg8_outer.cu:50-53)
   // waves per CU 0: one tile per wave
@@ -229,18 +234,23 @@ extern "C" int mrg8_batch_autotune(mrg8_batch_t b, void* stream, const char* cac
   for (int bt : blocks)
     for (int w : waves_per_cu)
       for (uint64_t t : tiles) {
+        if (rc) break;
         const mrg8_batch_config c{bt, w / (bt / 64), t};
         if ((w > 0 && c.blocks_per_cu < 1) || validate_config(c) != MRG8_OK) continue;
-        if (count_tiles(b->slot.segs(), b->slot.staged_count(), t) > b->max_tiles) continue;
+        const uint64_t nt = count_tiles(b->slot.segs(), b->slot.staged_count(), t);
+        if (nt > b->max_tiles) continue;
+        if (w == 0 && (nt + bt / 64 - 1) / (bt / 64) * (uint64_t)bt > 0xffffffffull) continue;  // 32-bit grid
         if ((rc = mrg8_batch_set_config(b, &c)) != MRG8_OK) break;
         if ((rc = mrg8_batch_run(b, s)) != MRG8_OK) break;  // warm-up (and capture)
-        (void)hipEventRecord(e0, s);
+        hipError_t e = hipEventRecord(e0, s);
         const int reps = 3;
         for (int r = 0; r < reps && rc == MRG8_OK; ++r) rc = mrg8_batch_run(b, s);
-        (void)hipEventRecord(e1, s);
-        (void)hipEventSynchronize(e1);
+        if (e == hipSuccess) e = hipEventRecord(e1, s);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
         float ms = 0;
-        (void)hipEventElapsedTime(&ms, e0, e1);
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        if (rc == MRG8_OK && e != hipSuccess) rc = hip_fail(e, "autotune timing");
+        if (rc) break;
         ms /= reps;
         if (ms < best_ms) {
           best_ms = ms;

@@ -179,7 +179,11 @@ int fill_device(DeviceContext& ctx, const uint32_t state[8], uint64_t offset, vo
   int def_block = 0;
   uint64_t def_tile = 0;
   const int cls = main_n ? 63 - __builtin_clzll(main_n) : 0;
-  const TunedShape& tuned = ctx.tuned[format][cls];
+  TunedShape tuned;
+  {
+    std::lock_guard<std::mutex> lk(ctx.tuned_mu);
+    tuned = ctx.tuned[format][cls];
+  }
   if (tuned.valid && !per_lane && (tuned.per_lane == 1 || (reinterpret_cast<uintptr_t>(main_ptr) & 15u) == 0)) {
     pl = tuned.per_lane;
     def_block = tuned.block_threads;
@@ -193,6 +197,11 @@ int fill_device(DeviceContext& ctx, const uint32_t state[8], uint64_t offset, vo
   const uint64_t quantum = 64ull * pl;
   if (tile == 0) tile = def_tile;
   if (tile % quantum) tile = (tile + quantum - 1) / quantum * quantum;
+  // The grid (work-items in total) is a 32-bit quantity.
+  const uint64_t wpb = (uint64_t)block_threads / 64;
+  if (((main_n + tile - 1) / tile + wpb - 1) / wpb * (uint64_t)block_threads > 0xffffffffull)
+    return fail(MRG8_ERR_ARG, "%llu outputs at %llu per wave need more than 2^32 - 1 work-items; use more outputs per wave",
+                (unsigned long long)main_n, (unsigned long long)tile);
   FillArgs a{};
   a.out = main_ptr;
   a.n = main_n;
@@ -220,8 +229,8 @@ struct mrg8_gen {
   DeviceContext* ctx = nullptr;
   uint32_t state[8] = {0};
   uint32_t seed = 0;
-  hipStream_t last_stream = nullptr;
-  bool issued = false;
+  // streams generated on since the last mrg8_synchronize
+  std::vector<hipStream_t> streams;
   // host-memory forms: two device chunks, two pinned host chunks, a stream
   // and one event per chunk (allocated on first use)
   void* dev_chunk[2] = {nullptr, nullptr};
@@ -298,12 +307,11 @@ extern "C" int mrg8_create(uint32_t seed, mrg8_gen_t* out) {
 extern "C" int mrg8_destroy(mrg8_gen_t g) {
   clear_error();
   if (!g) return MRG8_OK;
-  if (g->issued) {
-    (void)hipSetDevice(g->ctx->device);
-    (void)hipStreamSynchronize(g->last_stream);
-  }
+  // Kernels in flight read only their arguments and the device tables, which
+  // live until the process ends: nothing of `g` needs to outlive this call
+  // except the host-form staging, which is drained first.
   if (g->host_stream) {
-    (void)hipSetDevice(g->ctx->device);
+    DeviceGuard dg(g->ctx->device);
     release_host_staging(g);
   }
   delete g;
@@ -409,8 +417,7 @@ static int generate_impl(mrg8_gen_t g, void* dptr, uint64_t n, int format, void*
   hipStream_t s = static_cast<hipStream_t>(stream);
   int rc = fill_device(*g->ctx, g->state, 0, dptr, n, format, s, block_threads, tile, per_lane);
   if (rc) return rc;
-  g->last_stream = s;
-  g->issued = true;
+  if (std::find(g->streams.begin(), g->streams.end(), s) == g->streams.end()) g->streams.push_back(s);
   jump64(host_jump_table(), n, g->state);  // the state after n outputs
   return MRG8_OK;
 }
@@ -429,13 +436,13 @@ extern "C" int mrg8_generate_shape(mrg8_gen_t g, void* dptr, uint64_t n, int for
 extern "C" int mrg8_synchronize(mrg8_gen_t g) {
   clear_error();
   if (!g) return fail(MRG8_ERR_ARG, "null generator");
-  if (!g->issued) return MRG8_OK;
-  int prev = 0;
-  (void)hipGetDevice(&prev);
-  (void)hipSetDevice(g->ctx->device);
-  hipError_t e = hipStreamSynchronize(g->last_stream);
-  (void)hipSetDevice(prev);
-  if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+  if (g->streams.empty()) return MRG8_OK;
+  DeviceGuard dg(g->ctx->device);
+  for (hipStream_t s : g->streams) {
+    hipError_t e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+  }
+  g->streams.clear();
   return MRG8_OK;
 }
 
@@ -521,8 +528,6 @@ static int host_fill(mrg8_gen_t g, void* host, uint64_t n, int format) {
       return hip_fail(e, "host staging allocation");
     }
   }
-  // Finish earlier stream-ordered work on this generator first.
-  if (g->issued) (void)hipStreamSynchronize(g->last_stream);
   const uint64_t per = chunk_bytes / esz;
   const uint64_t nchunks = (n + per - 1) / per;
   auto count = [&](uint64_t i) { return std::min<uint64_t>(per, n - i * per); };
@@ -715,7 +720,10 @@ extern "C" int mrg8_tune_fill(int device, int format, uint64_t n, const char* ca
       }
     }
   }
-  ctx->tuned[format][cls] = found;
+  {
+    std::lock_guard<std::mutex> lk(ctx->tuned_mu);
+    ctx->tuned[format][cls] = found;
+  }
   if (best) {
     best->block_threads = found.block_threads;
     best->outputs_per_lane = found.per_lane;

@@ -505,7 +505,7 @@ hipError_t fill_one(const FillArgs& a, int block_threads, hipStream_t stream) {
   uint64_t blocks = (waves + wpb - 1) / wpb;
   if (blocks == 0 && a.peel) blocks = 1;  // only the peeled outputs
   if (blocks == 0) return hipSuccess;
-  if (blocks > 0x7fffffffull) return hipErrorInvalidConfiguration;
+  if (blocks * (uint64_t)block_threads > 0xffffffffull) return hipErrorInvalidConfiguration;  // 32-bit grid
   hipLaunchKernelGGL((k_fill<FMT, PL>), dim3((uint32_t)blocks), dim3(block_threads), 0, stream, a);
   return hipGetLastError();
 }
@@ -536,7 +536,7 @@ hipError_t launch_batch_tiles(const BatchArgs& a, uint64_t ntiles, int block_thr
   const uint64_t wpb = (uint64_t)block_threads / 64;
   const uint64_t blocks = (ntiles + wpb - 1) / wpb;
   if (blocks == 0) return hipSuccess;
-  if (blocks > 0x7fffffffull) return hipErrorInvalidConfiguration;
+  if (blocks * (uint64_t)block_threads > 0xffffffffull) return hipErrorInvalidConfiguration;  // 32-bit grid
   hipLaunchKernelGGL(k_batch_tiles, dim3((uint32_t)blocks), dim3(block_threads), 0, stream, a);
   return hipGetLastError();
 }

@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <mutex>
 
 #include "mrg8_device.hpp"
 
@@ -23,6 +24,7 @@ struct DeviceContext {
   int num_cu = 0;
   DeviceTables tables;
   TunedShape tuned[3][64];  // [format][floor(log2 n)], set by mrg8_tune_fill
+  std::mutex tuned_mu;      // guards `tuned` (fills on other handles read it)
 };
 
 // Records `fmt` as this thread's last error message and returns `code`.

@@ -137,6 +137,12 @@ int BatchSlot::capture(const LaunchConfig& cfg) {
 int BatchSlot::run(const LaunchConfig& cfg, hipStream_t stream, bool direct) {
   if (count_ == 0 || ntiles_ == 0) return MRG8_OK;
   if (ntiles_ > max_tiles_) return fail(MRG8_ERR_CAPACITY, "staged work needs more than max_tiles tiles");
+  if (cfg.blocks_per_cu == 0) {  // one wave per tile: the grid (work-items in total) is a 32-bit quantity
+    const uint64_t wpb = (uint64_t)cfg.block_threads / 64;
+    if ((ntiles_ + wpb - 1) / wpb * (uint64_t)cfg.block_threads > 0xffffffffull)
+      return fail(MRG8_ERR_CAPACITY, "%llu tiles need more than 2^32 - 1 work-items at one tile per wave; use larger "
+                  "tiles or a persistent grid", (unsigned long long)ntiles_);
+  }
   DeviceGuard dg(ctx_->device);
   int rc = direct ? MRG8_OK : capture(cfg);
   if (rc) return rc;

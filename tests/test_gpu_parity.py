@@ -342,6 +342,32 @@ def test_threads_with_own_handles_and_streams(gpu, oracle_mod):
         assert np.array_equal(results[t].view(np.uint8), want.view(np.uint8)), t
 
 
+def test_one_handle_on_two_streams(gpu, oracle_mod):
+    """Successive calls on two streams continue the sequence; synchronize
+    waits for both, and a handle destroyed with work in flight leaves that
+    work to finish on its stream."""
+    from mrg8_amd import hip
+
+    n = 1 << 20
+    say("two streams")
+    a, b = hip.Stream(), hip.Stream()
+    bufs = [hip.DeviceBuffer(n * 8) for _ in range(3)]
+    g = gpu.mrg8_cuda(SEED)
+    g.generate(bufs[0], n, 0, stream=a)
+    g.generate(bufs[1], n, 0, stream=b)
+    g.synchronize()
+    g.generate(bufs[2], n, 0, stream=b)
+    g.close()  # no wait: the launch completes on stream b
+    b.synchronize()
+    want, _ = oracle_mod.generate(oracle_mod.seed_state(SEED), 3 * n, 0)
+    for i, buf in enumerate(bufs):
+        got = buf.copy_to_host(np.float64, n)
+        assert np.array_equal(got.view(np.uint64), want[i * n : (i + 1) * n].view(np.uint64)), i
+        buf.free()
+    a.destroy()
+    b.destroy()
+
+
 def test_generate_inside_a_hip_graph(gpu, oracle_mod):
     """A generate call captured into a HIP graph (torch.cuda.graph) replays
     the positions it had at capture: each replay rewrites the same outputs,
