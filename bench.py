@@ -14,7 +14,7 @@ The same JSON line carries, measured in the same run:
   u32           the same workload in raw uint32 words
   small         2^24 float64 outputs (the reference's full-compare size)
   batched       4096 streams, log-uniform lengths in [2^10, 2^22), float64, one
-                persistent graph-captured launch with the autotuned shape
+                launch with the autotuned shape
   jumps         65536 streams jumped by uniform 63-bit steps, 256 uint32 each
   served        1e5 Poisson requests on 16384 resident streams, sizes
                 log-uniform in [2^8, 2^26) float64, 64 GB admission cap:
@@ -184,7 +184,7 @@ def bench_batched(args):
     log(f"batched: {total / (km * 1e-3) / 1e9:.1f} G samples/s, {gbs:.0f} GB/s, cfg {cfg}")
     return {"streams": len(lengths), "outputs": total, "ms_per_launch": km, "gsps": total / (km * 1e-3) / 1e9,
             "gbs": gbs, "frac_of_peak": gbs / PEAK_GBS, "config": cfg,
-            "launch": "hipGraph replay, " + ("one tile per wave" if cfg["blocks_per_cu"] == 0 else "persistent grid")}
+            "launch": "direct launch, " + ("one tile per wave" if cfg["blocks_per_cu"] == 0 else "persistent grid")}
 
 
 def bench_jumps(args):

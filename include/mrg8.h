@@ -199,17 +199,23 @@ typedef struct mrg8_batch_config {
 } mrg8_batch_config;
 
 /* A batch object owns device descriptor and tile tables sized for
- * `max_streams` segments and `max_tiles` work items, and a HIP graph holding
- * the persistent launch (no reference counterpart: the reference generates
- * one stream per blocking call). */
+ * `max_streams` segments and `max_tiles` work items (no reference
+ * counterpart: the reference generates one stream per blocking call). */
 int mrg8_batch_create(int device, uint32_t max_streams, uint64_t max_tiles, mrg8_batch_t* out);
 int mrg8_batch_destroy(mrg8_batch_t b);
 int mrg8_batch_get_config(mrg8_batch_t b, mrg8_batch_config* cfg);
 int mrg8_batch_set_config(mrg8_batch_t b, const mrg8_batch_config* cfg);
 /* Stages `count` descriptors (host memory) for the next run. */
 int mrg8_batch_set(mrg8_batch_t b, const mrg8_stream_desc* descs, uint32_t count);
-/* Replays the captured launch on `stream` over the staged descriptors. */
+/* Launches the staged descriptors on `stream` (stream-ordered; the staged
+ * block is copied to the device only when it changed).  The batch remembers
+ * the stream of its last run: destroy the batch, or run it on another
+ * stream, before destroying that stream. */
 int mrg8_batch_run(mrg8_batch_t b, void* stream);
+/* Runs replay a captured HIP graph holding the launch (enable != 0) instead
+ * of launching the kernel directly (the default: on MI355X a one-kernel
+ * graph replay costs ~10 us more per launch). */
+int mrg8_batch_use_graph(mrg8_batch_t b, int enable);
 /* Times candidate configurations on the staged descriptors (the outputs are
  * rewritten each trial) and keeps the fastest.  `cache_path` (may be NULL)
  * is a text file of results keyed by device and workload class. */

@@ -199,7 +199,8 @@ def write_jump_matrix(path: str) -> None:
 
 
 class Batch:
-    """Many stream segments generated by one persistent, graph-captured launch."""
+    """Many stream segments generated by one launch (a plain kernel launch,
+    or with use_graph(True) a replay of a captured HIP graph)."""
 
     def __init__(self, max_streams: int, max_tiles: int = 1 << 21, device: int = 0):
         h = ctypes.c_void_p()
@@ -244,6 +245,9 @@ class Batch:
 
     def run(self, stream=None) -> None:
         capi.call("mrg8_batch_run", self._h, _stream(stream))
+
+    def use_graph(self, enable: bool = True) -> None:
+        capi.call("mrg8_batch_use_graph", self._h, 1 if enable else 0)
 
     def autotune(self, stream=None, cache_path: str | None = None):
         c = capi.BatchConfig()

@@ -419,8 +419,6 @@ __global__ __launch_bounds__(1024) void k_batch(BatchArgs a) {
   const uint64_t nwaves = (uint64_t)gridDim.x * (blockDim.x >> 6);
   const uint64_t wave = (uint64_t)blockIdx.x * (blockDim.x >> 6) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   if (wave >= ntiles) return;
-  Coef k[2];
-  load_lane_coefs<0, 2>(k, a.lane);  // the PL = 2 layout is the same for every format
   const uint64_t mine_total = (ntiles - wave + nwaves - 1) / nwaves;
   for (uint64_t j0 = 0; j0 < mine_total; j0 += kGroup) {
     const uint32_t ngrp = (uint32_t)min<uint64_t>(kGroup, mine_total - j0);
@@ -430,6 +428,9 @@ __global__ __launch_bounds__(1024) void k_batch(BatchArgs a) {
 #pragma unroll
     for (int j = 0; j < 8; ++j) s[j] = ti.state[j];
     jump_groups(s, mine ? ti.offset : 0, a.jump);
+    // loaded after the jumps so that they hold no registers during them
+    Coef k[2];
+    load_lane_coefs<0, 2>(k, a.lane);  // the PL = 2 layout is the same for every format
     for (uint32_t g = 0; g < ngrp; ++g) {
       uint32_t u[8];
 #pragma unroll

@@ -234,7 +234,7 @@ int launch_steps(mrg8_server* s) {
                     s->active.end());
     if (n == 0) return MRG8_OK;
     step->slot.finalise(n, s->launch.tile);
-    if ((rc = step->slot.run(s->launch, step->stream, /*direct=*/true)) != MRG8_OK) {
+    if ((rc = step->slot.run(s->launch, step->stream, /*graph=*/false, /*track=*/true)) != MRG8_OK) {
       // nothing of this step was launched: its chunks go back to their requests
       for (auto& [r, take] : taken) {
         if (r->st == RState::Issued) {

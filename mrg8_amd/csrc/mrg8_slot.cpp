@@ -42,7 +42,7 @@ void BatchSlot::release() {
   if (!ctx_) return;
   DeviceGuard dg(ctx_->device);
   if (copy_pending_ && staged_) (void)hipEventSynchronize(staged_);
-  if (run_pending_ && run_done_) (void)hipEventSynchronize(run_done_);
+  if (run_pending_ && run_done_) (void)wait_done();
   if (exec_) (void)hipGraphExecDestroy(exec_);
   if (graph_) (void)hipGraphDestroy(graph_);
   if (staged_) (void)hipEventDestroy(staged_);
@@ -54,6 +54,8 @@ void BatchSlot::release() {
   staged_ = run_done_ = nullptr;
   hmem_ = dmem_ = nullptr;
   copy_pending_ = run_pending_ = have_graph_ = false;
+  last_stream_ = nullptr;
+  last_marked_ = true;
   ctx_ = nullptr;
 }
 
@@ -110,7 +112,7 @@ int BatchSlot::capture(const LaunchConfig& cfg) {
   if (have_graph_ && cfg.block_threads == captured_.block_threads && cfg.blocks_per_cu == captured_.blocks_per_cu &&
       cfg.tile == captured_.tile && (cfg.blocks_per_cu > 0 || captured_ntiles_ == ntiles_))
     return MRG8_OK;
-  if (run_pending_) (void)hipEventSynchronize(run_done_);
+  if (int rc = wait_done()) return rc;  // the old graph may still be running
   if (exec_) (void)hipGraphExecDestroy(exec_);
   if (graph_) (void)hipGraphDestroy(graph_);
   exec_ = nullptr;
@@ -134,7 +136,14 @@ int BatchSlot::capture(const LaunchConfig& cfg) {
   return MRG8_OK;
 }
 
-int BatchSlot::run(const LaunchConfig& cfg, hipStream_t stream, bool direct) {
+hipError_t BatchSlot::mark_last_run() {
+  if (!run_pending_ || last_marked_) return hipSuccess;
+  hipError_t e = hipEventRecord(run_done_, last_stream_);
+  if (e == hipSuccess) last_marked_ = true;
+  return e;
+}
+
+int BatchSlot::run(const LaunchConfig& cfg, hipStream_t stream, bool graph, bool track) {
   if (count_ == 0 || ntiles_ == 0) return MRG8_OK;
   if (ntiles_ > max_tiles_) return fail(MRG8_ERR_CAPACITY, "staged work needs more than max_tiles tiles");
   if (cfg.blocks_per_cu == 0) {  // one wave per tile: the grid (work-items in total) is a 32-bit quantity
@@ -144,30 +153,36 @@ int BatchSlot::run(const LaunchConfig& cfg, hipStream_t stream, bool direct) {
                   "tiles or a persistent grid", (unsigned long long)ntiles_);
   }
   DeviceGuard dg(ctx_->device);
-  int rc = direct ? MRG8_OK : capture(cfg);
+  int rc = graph ? capture(cfg) : MRG8_OK;
   if (rc) return rc;
   hipError_t e = hipSuccess;
-  if (dirty_) {  // replays of an unchanged batch copy nothing
+  if (dirty_) {  // runs of an unchanged batch copy nothing
     // The device block may be overwritten once this slot's previous launch
     // has finished: on the same stream the copy is ordered after it; on
     // another stream it waits for it.
-    if (run_pending_) e = hipStreamWaitEvent(stream, run_done_, 0);
+    if (run_pending_ && stream != last_stream_) {
+      if (mark_last_run() == hipSuccess) e = hipStreamWaitEvent(stream, run_done_, 0);
+      else if ((rc = wait_done()) != MRG8_OK) return rc;  // the last stream no longer accepts work
+    }
     if (e == hipSuccess) e = hipMemcpyAsync(dmem_, hmem_, used_bytes_, hipMemcpyHostToDevice, stream);
     if (e == hipSuccess) e = hipEventRecord(staged_, stream);
     if (e != hipSuccess) return hip_fail(e, "batch staging copy");
     copy_pending_ = true;
     dirty_ = false;
   }
-  e = direct ? launch(cfg, stream) : hipGraphLaunch(exec_, stream);
-  if (e == hipSuccess) e = hipEventRecord(run_done_, stream);
-  if (e != hipSuccess) return hip_fail(e, direct ? "batch launch" : "hipGraphLaunch");
+  e = graph ? hipGraphLaunch(exec_, stream) : launch(cfg, stream);
+  if (e != hipSuccess) return hip_fail(e, graph ? "hipGraphLaunch" : "batch launch");
   run_pending_ = true;
+  last_stream_ = stream;
+  last_marked_ = false;
+  if (track && (e = mark_last_run()) != hipSuccess) return hip_fail(e, "hipEventRecord(run)");
   return MRG8_OK;
 }
 
 bool BatchSlot::done() {
   if (!run_pending_) return true;
   DeviceGuard dg(ctx_->device);
+  if (mark_last_run() != hipSuccess) return false;
   if (hipEventQuery(run_done_) == hipSuccess) {
     run_pending_ = false;
     copy_pending_ = false;
@@ -179,8 +194,9 @@ bool BatchSlot::done() {
 int BatchSlot::wait_done() {
   if (!run_pending_) return MRG8_OK;
   DeviceGuard dg(ctx_->device);
-  hipError_t e = hipEventSynchronize(run_done_);
-  if (e != hipSuccess) return hip_fail(e, "hipEventSynchronize(run)");
+  hipError_t e = mark_last_run();
+  e = e == hipSuccess ? hipEventSynchronize(run_done_) : hipDeviceSynchronize();
+  if (e != hipSuccess) return hip_fail(e, "waiting for the last batch run");
   run_pending_ = false;
   copy_pending_ = false;
   return MRG8_OK;

@@ -1,13 +1,19 @@
-// A batch slot: everything one in-flight persistent launch needs.
+// A batch slot: everything one in-flight batched launch needs.
 //
 // One device block holds the batch block of mrg8_device.hpp (header,
 // segments, tile prefix sums, tile -> segment map), sized for `cap` segments
 // and `max_tiles` tiles; a pinned host block of the same size stages it.  A
 // run copies the used prefix host-to-device in one copy when it changed since
-// the last run, on the launch stream, and replays a captured graph holding
-// the persistent kernel (or launches the kernel directly); the kernel reads
-// the counts and table offsets from the header, so the graph is captured once
-// per launch configuration.
+// the last run, on the launch stream, and launches the kernel directly or
+// replays a captured graph holding it; the kernel reads the counts and table
+// offsets from the header, so a graph is captured once per launch
+// configuration.
+//
+// Completion of a run is an event recorded behind it on its stream.  A
+// tracked run records it at once (the serving loop polls it); otherwise it
+// is recorded only when something needs it -- a copy issued on another
+// stream, a wait, a re-capture, release -- since every event record is one
+// more packet the stream must process between kernels.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -49,17 +55,20 @@ class BatchSlot {
   uint64_t staged_tiles() const { return ntiles_; }
   uint32_t staged_count() const { return count_; }
 
-  // Copies the staged block (when it changed) and launches on `stream`: by
-  // replaying the captured graph, or with `direct` by a plain kernel launch
-  // (cheaper on the host for a launch whose block changes every time).
-  int run(const LaunchConfig& cfg, hipStream_t stream, bool direct = false);
+  // Copies the staged block (when it changed) and launches on `stream`:
+  // a plain kernel launch, or with `graph` a replay of the captured graph;
+  // with `track` the run's completion event is recorded at once.
+  int run(const LaunchConfig& cfg, hipStream_t stream, bool graph, bool track);
   // The last run finished (non-blocking query).
   bool done();
+  // Waits for the last run.  If the stream it ran on no longer accepts
+  // work, waits for the whole device.
   int wait_done();
-  hipEvent_t done_event() const { return run_done_; }
 
  private:
   int capture(const LaunchConfig& cfg);
+  // Records the completion event behind the last run if it is not there yet.
+  hipError_t mark_last_run();
   BatchArgs args(const LaunchConfig& cfg) const;
   hipError_t launch(const LaunchConfig& cfg, hipStream_t stream) const;
 
@@ -74,6 +83,8 @@ class BatchSlot {
   BatchSegment* h_segs_ = nullptr;
   hipEvent_t staged_ = nullptr, run_done_ = nullptr;
   bool copy_pending_ = false, run_pending_ = false, dirty_ = false;
+  hipStream_t last_stream_ = nullptr;  // stream of the last run
+  bool last_marked_ = true;            // run_done_ is recorded behind the last run
   uint32_t count_ = 0;
   uint64_t ntiles_ = 0;
   hipGraph_t graph_ = nullptr;

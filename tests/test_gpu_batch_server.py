@@ -39,9 +39,10 @@ class Packed:
         return self.buf.copy_to_host(DT[self.fmts[i]], int(self.counts[i]), self.offsets[i])
 
 
+@pytest.mark.parametrize("graph", [False, True], ids=["direct", "graph"])
 @pytest.mark.parametrize("shapes", [((256, 4, 4096), (512, 2, 16384)), ((1024, 0, 4096), (256, 0, 512))],
                          ids=["persistent", "tile-per-wave"])
-def test_batch_mixed_segments(gpu, oracle_mod, shapes):
+def test_batch_mixed_segments(gpu, oracle_mod, shapes, graph):
     rng = np.random.default_rng(21)
     n = 200
     states = rng.integers(0, 2**31, size=(n, 8)).astype(np.uint32)
@@ -53,11 +54,12 @@ def test_batch_mixed_segments(gpu, oracle_mod, shapes):
     offs[:20] = list(range(20))  # small offsets take the shift rows of the jump
     pk = Packed(counts, fmts, gap_elems=int(rng.integers(1, 4)))
     b = gpu.Batch(max_streams=n, max_tiles=1 << 20)
+    b.use_graph(graph)
     b.set_config(*shapes[0])
     segs = [(states[i], offs[i], int(counts[i]), pk.ptr(i), int(fmts[i])) for i in range(n)]
     b.set(segs)
-    for rep in range(3):  # graph replays give identical results
-        say("batch-mixed", rep)
+    for rep in range(3):  # repeated runs (graph replays) give identical results
+        say("batch-mixed", shapes, graph, rep)
         b.run()
         gpu.hip.synchronize()
         for i in range(n):
@@ -67,6 +69,41 @@ def test_batch_mixed_segments(gpu, oracle_mod, shapes):
             got = pk.fetch(i)
             assert np.array_equal(got.view(np.uint8), want.view(np.uint8)), (rep, i)
         b.set_config(*shapes[1])  # a new shape, same outputs
+
+
+@pytest.mark.parametrize("graph", [False, True], ids=["direct", "graph"])
+def test_batch_restaged_on_another_stream(gpu, oracle_mod, graph):
+    """A batch run on one stream, re-staged and run at once on another: the
+    new block's copy waits for the first run, and both runs' outputs hold."""
+    from mrg8_amd import hip
+
+    rng = np.random.default_rng(5)
+    n = 64
+    st = rng.integers(1, 2**31 - 1, size=(2, n, 8)).astype(np.uint32)
+    cnt = rng.integers(1000, 300000, size=(2, n))
+    offs = rng.integers(0, 2**40, size=(2, n), dtype=np.uint64)
+    bufs = [hip.DeviceBuffer(int(cnt[r].sum()) * 8) for r in range(2)]
+    starts = [np.concatenate([[0], np.cumsum(cnt[r])[:-1]]) for r in range(2)]
+    a_s, b_s = hip.Stream(), hip.Stream()
+    b = gpu.Batch(max_streams=n, max_tiles=1 << 16)
+    b.use_graph(graph)
+    say("restaged", graph)
+    for r, s in ((0, a_s), (1, b_s)):
+        b.set([(st[r, i], int(offs[r, i]), int(cnt[r, i]), bufs[r].ptr + 8 * int(starts[r][i]), 0) for i in range(n)])
+        b.run(s)
+    b_s.synchronize()
+    a_s.synchronize()
+    for r in range(2):
+        got = bufs[r].copy_to_host(np.float64, int(cnt[r].sum()))
+        for i in range(0, n, 7):
+            want, _ = oracle_mod.generate(oracle_mod.jump(st[r, i], int(offs[r, i])), int(cnt[r, i]), 0)
+            part = got[int(starts[r][i]) : int(starts[r][i]) + int(cnt[r, i])]
+            assert np.array_equal(part.view(np.uint64), want.view(np.uint64)), (r, i)
+    b.close()
+    for x in bufs:
+        x.free()
+    a_s.destroy()
+    b_s.destroy()
 
 
 @pytest.mark.parametrize("bpc", [1, 0], ids=["persistent", "tile-per-wave"])
