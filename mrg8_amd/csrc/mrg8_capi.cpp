@@ -231,9 +231,8 @@ struct mrg8_gen {
   uint32_t seed = 0;
   // streams generated on since the last mrg8_synchronize
   std::vector<hipStream_t> streams;
-  // host-memory forms: two device chunks, two pinned host chunks, a stream
-  // and one event per chunk (allocated on first use)
-  void* dev_chunk[2] = {nullptr, nullptr};
+  // host-memory forms: two pinned host chunks, a stream and one event per
+  // chunk (allocated on first use)
   void* pin_chunk[2] = {nullptr, nullptr};
   hipEvent_t chunk_ready[2] = {nullptr, nullptr};
   hipStream_t host_stream = nullptr;
@@ -244,10 +243,9 @@ namespace {
 void release_host_staging(mrg8_gen* g) {
   if (g->host_stream) (void)hipStreamSynchronize(g->host_stream);
   for (int i = 0; i < 2; ++i) {
-    if (g->dev_chunk[i]) (void)hipFree(g->dev_chunk[i]);
     if (g->pin_chunk[i]) (void)hipHostFree(g->pin_chunk[i]);
     if (g->chunk_ready[i]) (void)hipEventDestroy(g->chunk_ready[i]);
-    g->dev_chunk[i] = g->pin_chunk[i] = nullptr;
+    g->pin_chunk[i] = nullptr;
     g->chunk_ready[i] = nullptr;
   }
   if (g->host_stream) (void)hipStreamDestroy(g->host_stream);
@@ -505,9 +503,13 @@ static void parallel_copy(void* dst, const void* src, size_t bytes) {
   for (auto& t : th) t.join();
 }
 
-// Host-memory forms: generate into device chunks and bring them back through
-// pinned chunks, two of each in flight: while the host copies chunk i out of
-// its pinned buffer, the device generates and transfers chunk i + 1.
+// Host-memory forms: the kernel writes each chunk straight into a pinned host
+// buffer over PCIe, two chunks in flight: while the host copies chunk i into
+// the caller's memory, the device writes chunk i + 1.  Measured on MI355X:
+// ~51 GB/s for the kernel's own writes to host memory whatever stream it runs
+// on, where a device chunk followed by hipMemcpyAsync ran at 31-54 GB/s
+// depending on the stream.  The chunks are coherent host memory, which the
+// host reads back about 4x faster than the default kind after device writes.
 static int host_fill(mrg8_gen_t g, void* host, uint64_t n, int format) {
   if (!g) return fail(MRG8_ERR_ARG, "null generator");
   if (n == 0) return MRG8_OK;
@@ -519,8 +521,7 @@ static int host_fill(mrg8_gen_t g, void* host, uint64_t n, int format) {
   if (!g->host_stream) {
     e = hipStreamCreateWithFlags(&g->host_stream, hipStreamNonBlocking);
     for (int i = 0; i < 2 && e == hipSuccess; ++i) {
-      e = hipMalloc(&g->dev_chunk[i], chunk_bytes);
-      if (e == hipSuccess) e = hipHostMalloc(&g->pin_chunk[i], chunk_bytes, hipHostMallocDefault);
+      e = hipHostMalloc(&g->pin_chunk[i], chunk_bytes, hipHostMallocCoherent);
       if (e == hipSuccess) e = hipEventCreateWithFlags(&g->chunk_ready[i], hipEventDisableTiming);
     }
     if (e != hipSuccess) {
@@ -534,12 +535,11 @@ static int host_fill(mrg8_gen_t g, void* host, uint64_t n, int format) {
   auto issue = [&](uint64_t i) -> int {
     const int b = (int)(i & 1);
     const uint64_t m = count(i);
-    int rc = fill_device(*g->ctx, g->state, 0, g->dev_chunk[b], m, format, g->host_stream, 0, 0, 0);
+    int rc = fill_device(*g->ctx, g->state, 0, g->pin_chunk[b], m, format, g->host_stream, 0, 0, 0);
     if (rc) return rc;
     jump64(host_jump_table(), m, g->state);
-    hipError_t x = hipMemcpyAsync(g->pin_chunk[b], g->dev_chunk[b], m * esz, hipMemcpyDeviceToHost, g->host_stream);
-    if (x == hipSuccess) x = hipEventRecord(g->chunk_ready[b], g->host_stream);
-    return x == hipSuccess ? MRG8_OK : hip_fail(x, "hipMemcpyAsync(D2H)");
+    hipError_t x = hipEventRecord(g->chunk_ready[b], g->host_stream);
+    return x == hipSuccess ? MRG8_OK : hip_fail(x, "hipEventRecord(chunk)");
   };
   int rc = issue(0);
   if (rc == MRG8_OK && nchunks > 1) rc = issue(1);
@@ -547,7 +547,7 @@ static int host_fill(mrg8_gen_t g, void* host, uint64_t n, int format) {
     const int b = (int)(i & 1);
     e = hipEventSynchronize(g->chunk_ready[b]);
     if (e != hipSuccess) {
-      rc = hip_fail(e, "hipEventSynchronize(D2H)");
+      rc = hip_fail(e, "hipEventSynchronize(chunk)");
       break;
     }
     parallel_copy(static_cast<char*>(host) + i * per * esz, g->pin_chunk[b], count(i) * esz);
