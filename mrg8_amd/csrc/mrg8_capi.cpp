@@ -76,7 +76,7 @@ int device_context(int device, DeviceContext** out) {
   if ((e = hipSetDevice(device)) != hipSuccess) return hip_fail(e, "hipSetDevice");
 
   // Coefficient records: block rows (row 0 of A^(k+1)) and jump rows.
-  std::vector<CoefRecord> lane(kLaneRows), jump((size_t)kJumpPositions * kJumpValues * 8);
+  std::vector<CoefRecord> lane(kLaneRows), jump((size_t)kIdentityRec + 8);
   const std::vector<uint32_t> rows = block_rows(kLaneRows);
   for (int k = 0; k < kLaneRows; ++k) {
     if (!all_nonzero(&rows[(size_t)k * 8])) {
@@ -97,6 +97,11 @@ int device_context(int device, DeviceContext** out) {
       for (int r = 0; r < 8; ++r)
         jump[((size_t)k * kJumpValues + (d - 1)) * 8 + r] = make_record(&p[(size_t)r * 8]);
     }
+  }
+  for (int r = 0; r < 8; ++r) {  // the identity: a digit 0 leaves the state as it is
+    uint32_t row[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    row[r] = 1;
+    jump[(size_t)kIdentityRec + r] = make_record(row);
   }
 
   auto ctx = std::make_unique<DeviceContext>();

@@ -15,12 +15,15 @@ constexpr int kLaneRows = 256;   // block rows kept on the device: up to 4 outpu
 constexpr int kJumpBits = 8;
 constexpr int kJumpPositions = 64 / kJumpBits;
 constexpr int kJumpValues = (1 << kJumpBits) - 1;  // non-zero digit values
+// Record index of row 0 of the identity, stored after the digit matrices.
+constexpr int kIdentityRec = kJumpPositions * kJumpValues * 8;
 
 // Per-device constant tables.
 struct DeviceTables {
   CoefRecord* lane = nullptr;  // kLaneRows records: row 0 of A^(k+1), support 0xff
   // kJumpPositions * kJumpValues * 8 records: row r of A^(d 2^(kJumpBits k))
-  // at ((k * kJumpValues) + (d - 1)) * 8 + r
+  // at ((k * kJumpValues) + (d - 1)) * 8 + r; then the 8 rows of the
+  // identity at kIdentityRec + r
   CoefRecord* jump = nullptr;
 };
 
