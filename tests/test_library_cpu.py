@@ -121,6 +121,11 @@ def test_argument_errors(lib):
     assert L.mrg8_jump_matrix(None, 0) == lib.ERR_ARG
     buf = (ctypes.c_uint32 * 10)()
     assert L.mrg8_jump_matrix(buf, 10) == lib.ERR_ARG
+    # null handles are errors, never crashes
+    assert L.mrg8_batch_use_graph(None, 1) == lib.ERR_ARG
+    assert L.mrg8_batch_run(None, None) == lib.ERR_ARG
+    assert L.mrg8_synchronize(None) == lib.ERR_ARG
+    assert L.mrg8_destroy(None) == lib.OK
 
 
 def test_no_device_is_an_error_not_a_fallback(lib):
