@@ -174,6 +174,31 @@ int mrg8_rand1(mrg8_gen_t g, double* value);
 int mrg8_write_jump_matrix(const char* path);
 /* Copies the 247*64 table words (column-major per matrix) to `table`. */
 int mrg8_jump_matrix(uint32_t* table, size_t words);
+/* read_jump_matrix (original/mrg8.cpp:88-102): parses a jump_matrix.txt into
+ * the 247*64 words of `table` (layout of mrg8_jump_matrix).  A missing,
+ * short or malformed file is MRG8_ERR_IO (the reference exits).  The library
+ * never needs the file: its table is computed. */
+int mrg8_read_jump_matrix(const char* path, uint32_t* table, size_t words);
+/* Reads `path` and checks it word for word against the computed table:
+ * MRG8_OK, MRG8_ERR_IO (unreadable) or MRG8_ERR_STATE (a word differs). */
+int mrg8_check_jump_matrix(const char* path);
+
+/* ---- bulk jumps of device-resident states -------------------------------- */
+
+/* Advances n states on `device`: the 8 words at d_out + 8 i become those at
+ * d_in + 8 i jumped ahead by offset i, the little-endian integer of `words`
+ * 64-bit words at d_offsets + words i (words 1..4: offsets below 2^64 ..
+ * 2^256; the reference's widest jump, jump_ahead(short[200]) at
+ * original/mrg8.cpp:105-122 and cuda/mrg8.cpp:127-141, takes 200 bits).  The
+ * device-side counterpart of calling mrg8_advance_state_bits n times; the
+ * reference has no device jump past 32 bits (cuda/mrg8_cuda.cu:219,314).
+ * All pointers are device memory; d_in and d_out may be the same buffer
+ * (other overlaps are not allowed); state words must be at most 2^31 - 1.
+ * Stream-ordered on `stream`.  The first call with words > 1 builds a
+ * 2.3 MiB table for byte positions 8..31 on the device (kept for the
+ * process). */
+int mrg8_advance_states(int device, const uint32_t* d_in, const uint64_t* d_offsets, uint32_t words,
+                        uint32_t* d_out, uint64_t n, void* stream);
 
 /* ---- batched generation: many streams, one persistent launch ------------ */
 

@@ -198,6 +198,26 @@ def write_jump_matrix(path: str) -> None:
     capi.call("mrg8_write_jump_matrix", path.encode())
 
 
+def read_jump_matrix(path: str) -> np.ndarray:
+    """Parses a jump_matrix.txt (original/mrg8.cpp:88-102) into the layout of jump_matrix()."""
+    t = np.zeros(247 * 64, dtype=np.uint32)
+    capi.call("mrg8_read_jump_matrix", path.encode(), t.ctypes.data_as(capi.u32p), ctypes.c_size_t(t.size))
+    return t
+
+
+def check_jump_matrix(path: str) -> None:
+    """Raises Mrg8Error unless `path` holds exactly the computed table."""
+    capi.call("mrg8_check_jump_matrix", path.encode())
+
+
+def advance_states(d_in, d_offsets, words: int, d_out, n: int, stream=None, device: int = 0) -> None:
+    """Device-side bulk jump: n states (8 uint32 words each, device memory)
+    advanced by little-endian offsets of `words` uint64 words each (up to 256
+    bits); stream-ordered.  See mrg8_advance_states in include/mrg8.h."""
+    capi.call("mrg8_advance_states", device, ctypes.c_void_p(_ptr(d_in)), ctypes.c_void_p(_ptr(d_offsets)),
+              ctypes.c_uint32(words), ctypes.c_void_p(_ptr(d_out)), ctypes.c_uint64(n), _stream(stream))
+
+
 class Batch:
     """Many stream segments generated by one launch (a plain kernel launch,
     or with use_graph(True) a replay of a captured HIP graph)."""

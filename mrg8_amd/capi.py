@@ -99,6 +99,9 @@ SIGNATURES = {
     "mrg8_rand1": (ctypes.c_int, [H, ctypes.POINTER(ctypes.c_double)]),
     "mrg8_write_jump_matrix": (ctypes.c_int, [ctypes.c_char_p]),
     "mrg8_jump_matrix": (ctypes.c_int, [u32p, ctypes.c_size_t]),
+    "mrg8_read_jump_matrix": (ctypes.c_int, [ctypes.c_char_p, u32p, ctypes.c_size_t]),
+    "mrg8_check_jump_matrix": (ctypes.c_int, [ctypes.c_char_p]),
+    "mrg8_advance_states": (ctypes.c_int, [ctypes.c_int, vp, vp, ctypes.c_uint32, vp, ctypes.c_uint64, vp]),
     "mrg8_batch_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_uint32, ctypes.c_uint64, ctypes.POINTER(H)]),
     "mrg8_batch_destroy": (ctypes.c_int, [H]),
     "mrg8_batch_get_config": (ctypes.c_int, [H, ctypes.POINTER(BatchConfig)]),

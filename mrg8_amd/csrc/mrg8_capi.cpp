@@ -128,6 +128,39 @@ int device_context(int device, DeviceContext** out) {
   return MRG8_OK;
 }
 
+int ensure_wide_table(DeviceContext& ctx) {
+  std::lock_guard<std::mutex> lk(ctx.wide_mu);
+  if (ctx.tables.wide) return MRG8_OK;
+  // A^(d 2^(kJumpBits k)) for k = kJumpPositions..kWidePositions-1 and
+  // d = 1..kJumpValues; A^(2^j) past the last jump_matrix.txt entry by squaring.
+  const JumpTable& jt = host_jump_table();
+  const int npos = kWidePositions - kJumpPositions;
+  std::vector<CoefRecord> wide((size_t)npos * kJumpValues * 8);
+  for (int k = kJumpPositions; k < kWidePositions; ++k) {
+    const int j = kJumpBits * k;
+    std::vector<uint32_t> m(jt.at(std::min(j, kNumJump - 1)), jt.at(std::min(j, kNumJump - 1)) + 64);
+    for (int q = kNumJump - 1; q < j; ++q) m = mat_mul(m.data(), m.data());
+    std::vector<uint32_t> p = m;
+    for (int d = 1; d <= kJumpValues; ++d) {
+      if (d > 1) p = mat_mul(p.data(), m.data());
+      for (int r = 0; r < 8; ++r)
+        wide[((size_t)(k - kJumpPositions) * kJumpValues + (d - 1)) * 8 + r] = make_record(&p[(size_t)r * 8]);
+    }
+  }
+  DeviceGuard dg(ctx.device);
+  void* dev = nullptr;
+  const size_t bytes = sizeof(CoefRecord) * wide.size();
+  hipError_t e = hipMalloc(&dev, bytes);
+  if (e != hipSuccess) return hip_fail(e, "hipMalloc(wide jump table)");
+  e = hipMemcpy(dev, wide.data(), bytes, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    (void)hipFree(dev);
+    return hip_fail(e, "hipMemcpy(wide jump table)");
+  }
+  ctx.tables.wide = static_cast<CoefRecord*>(dev);
+  return MRG8_OK;
+}
+
 bool state_valid(const uint32_t st[8]) {
   for (int i = 0; i < 8; ++i)
     if (st[i] > kP) return false;
@@ -594,6 +627,40 @@ extern "C" int mrg8_rand1(mrg8_gen_t g, double* value) {
   return MRG8_OK;
 }
 
+// ---- bulk advance of device-resident states ----------------------------------------
+
+extern "C" int mrg8_advance_states(int device, const uint32_t* d_in, const uint64_t* d_offsets, uint32_t words,
+                                   uint32_t* d_out, uint64_t n, void* stream) {
+  clear_error();
+  if (n == 0) return MRG8_OK;
+  if (!d_in || !d_offsets || !d_out) return fail(MRG8_ERR_ARG, "null argument");
+  if (words < 1 || words > (uint32_t)kAdvanceWords)
+    return fail(MRG8_ERR_ARG, "words must be 1..%d (offsets of 64..%d bits)", kAdvanceWords, 64 * kAdvanceWords);
+  if ((reinterpret_cast<uintptr_t>(d_in) | reinterpret_cast<uintptr_t>(d_out)) & 3u ||
+      reinterpret_cast<uintptr_t>(d_offsets) & 7u)
+    return fail(MRG8_ERR_ARG, "states must be 4-byte and offsets 8-byte aligned");
+  if (n > UINT64_MAX / (8ull * words)) return fail(MRG8_ERR_ARG, "n too large");
+  DeviceContext* ctx = nullptr;
+  int rc = device_context(device, &ctx);
+  if (rc) return rc;
+  if (words > 1 && (rc = ensure_wide_table(*ctx)) != MRG8_OK) return rc;
+  DeviceGuard dg(device);
+  const uint64_t chunk = 1ull << 28;  // 8 lanes per state: 2^31 work-items per launch at most
+  for (uint64_t first = 0; first < n; first += chunk) {
+    AdvanceArgs a{};
+    a.in = d_in + 8 * first;
+    a.offsets = d_offsets + words * first;
+    a.out = d_out + 8 * first;
+    a.n = std::min(chunk, n - first);
+    a.words = words;
+    a.jump = ctx->tables.jump;
+    a.wide = ctx->tables.wide;
+    hipError_t e = launch_advance(a, static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return hip_fail(e, "advance launch");
+  }
+  return MRG8_OK;
+}
+
 // ---- jump table output -----------------------------------------------------------
 
 extern "C" int mrg8_jump_matrix(uint32_t* table, size_t words) {
@@ -604,6 +671,42 @@ extern "C" int mrg8_jump_matrix(uint32_t* table, size_t words) {
   for (int j = 0; j < kNumJump; ++j)
     for (int r = 0; r < 8; ++r)
       for (int c = 0; c < 8; ++c) table[(size_t)j * 64 + r + 8 * c] = t.at(j)[r * 8 + c];  // column-major
+  return MRG8_OK;
+}
+
+extern "C" int mrg8_read_jump_matrix(const char* path, uint32_t* table, size_t words) {
+  clear_error();
+  if (!path || !table) return fail(MRG8_ERR_ARG, "null argument");
+  if (words < (size_t)kNumJump * 64) return fail(MRG8_ERR_ARG, "table needs %d words", kNumJump * 64);
+  FILE* f = std::fopen(path, "r");
+  if (!f) return fail(MRG8_ERR_IO, "cannot open %s", path);
+  // original/mrg8.cpp:88-102: kNumJump lines of 64 unsigned words (column-major 8x8)
+  int rc = MRG8_OK;
+  for (size_t i = 0; i < (size_t)kNumJump * 64; ++i) {
+    unsigned long long v = 0;
+    if (std::fscanf(f, "%llu", &v) != 1) {
+      rc = fail(MRG8_ERR_IO, "%s: word %zu of %d missing or malformed", path, i, kNumJump * 64);
+      break;
+    }
+    if (v > kP) {
+      rc = fail(MRG8_ERR_IO, "%s: word %zu (%llu) is above 2^31 - 1", path, i, v);
+      break;
+    }
+    table[i] = (uint32_t)v;
+  }
+  std::fclose(f);
+  return rc;
+}
+
+extern "C" int mrg8_check_jump_matrix(const char* path) {
+  clear_error();
+  std::vector<uint32_t> file((size_t)kNumJump * 64), mine((size_t)kNumJump * 64);
+  int rc = mrg8_read_jump_matrix(path, file.data(), file.size());
+  if (rc) return rc;
+  mrg8_jump_matrix(mine.data(), mine.size());
+  for (size_t i = 0; i < file.size(); ++i)
+    if (file[i] != mine[i])
+      return fail(MRG8_ERR_STATE, "%s: word %zu is %u, A^(2^%zu) has %u", path, i, file[i], i / 64, mine[i]);
   return MRG8_OK;
 }
 

@@ -176,6 +176,28 @@ __device__ __forceinline__ const CoefRecord* digit_rec(const CoefRecord* __restr
   return jrec + (d ? (k * kJumpValues + d - 1) * 8 + row : kIdentityRec + row);
 }
 
+// One step of the pipelined jumps below: lane r of group g computes row r of
+// the group's matrix (record c) times the group's state, and the group's
+// eight new words are gathered back.
+__device__ __forceinline__ void jump_step(uint32_t s[8], const Coef& c, int grp) {
+  int32_t e[8];
+  centre(s, e);
+  uint32_t y = dot_residue(c, e) + 1u;
+  // bigDotProd is 0 when every product is 0; with all eight words non-zero
+  // that cannot happen (no row of a power of A is zero)
+  uint32_t smin = s[0];
+#pragma unroll
+  for (int j = 1; j < 8; ++j) smin = min(smin, s[j]);
+  if (smin == 0u) {
+    uint32_t nz = 0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) nz |= (s[j] != 0u ? 1u : 0u) << j;
+    if (!(c.support & nz)) y = 0u;
+  }
+#pragma unroll
+  for (int j = 0; j < 8; ++j) s[j] = (uint32_t)__builtin_amdgcn_ds_bpermute((grp * 8 + j) * 4, (int)y);
+}
+
 __device__ __forceinline__ void jump_groups_pipelined(uint32_t s[8], uint64_t off,
                                                       const CoefRecord* __restrict__ jrec) {
   const int lane = threadIdx.x & 63;
@@ -186,24 +208,7 @@ __device__ __forceinline__ void jump_groups_pipelined(uint32_t s[8], uint64_t of
     // wave-uniform: no group has a digit above position k
     const bool last = k + 1 == kJumpPositions || !__ballot((off >> (kJumpBits * (k + 1))) != 0);
     const Coef next = load_coef11(digit_rec(jrec, last ? 0 : off, last ? 0 : k + 1, row));
-    if (__ballot(digit_at(off, k) != 0)) {
-      int32_t e[8];
-      centre(s, e);
-      uint32_t y = dot_residue(c, e) + 1u;
-      // bigDotProd is 0 when every product is 0; with all eight words
-      // non-zero that cannot happen (no row of a power of A is zero)
-      uint32_t smin = s[0];
-#pragma unroll
-      for (int j = 1; j < 8; ++j) smin = min(smin, s[j]);
-      if (smin == 0u) {
-        uint32_t nz = 0;
-#pragma unroll
-        for (int j = 0; j < 8; ++j) nz |= (s[j] != 0u ? 1u : 0u) << j;
-        if (!(c.support & nz)) y = 0u;
-      }
-#pragma unroll
-      for (int j = 0; j < 8; ++j) s[j] = (uint32_t)__builtin_amdgcn_ds_bpermute((grp * 8 + j) * 4, (int)y);
-    }
+    if (__ballot(digit_at(off, k) != 0)) jump_step(s, c, grp);
     if (last) break;
     c = next;
   }
@@ -573,6 +578,56 @@ __global__ __launch_bounds__(1024) void k_batch_tiles(BatchArgs a) {
   }
 }
 
+// ---- bulk advance: many states, offsets of up to kAdvanceWords words --------
+
+// Digit at byte position k (wave-uniform) of a multiword offset.
+__device__ __forceinline__ uint32_t wide_digit(const uint64_t (&off)[kAdvanceWords], int k) {
+  uint64_t w = off[0];
+#pragma unroll
+  for (int i = 1; i < kAdvanceWords; ++i) w = (k >> 3) == i ? off[i] : w;
+  return (uint32_t)(w >> (8 * (k & 7))) & (uint32_t)kJumpValues;
+}
+
+__device__ __forceinline__ const CoefRecord* wide_rec(const AdvanceArgs& a, int k, uint32_t d, int row) {
+  if (d == 0u) return a.jump + (kIdentityRec + row);
+  return k < kJumpPositions ? a.jump + ((k * kJumpValues + (int)d - 1) * 8 + row)
+                            : a.wide + (((k - kJumpPositions) * kJumpValues + (int)d - 1) * 8 + row);
+}
+
+// Eight lanes per state (lane r of a group computes row r, as in the jumps
+// above); byte positions of the offset in order, positions where no state of
+// the wave has a digit skipped, the next position's records fetched while
+// the current step computes.  A state is read before it is written, so `in`
+// and `out` may be the same buffer.
+__global__ __launch_bounds__(256) void k_advance(AdvanceArgs a) {
+  const int lane = threadIdx.x & 63;
+  const int grp = lane >> 3, row = lane & 7;
+  const uint64_t item = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 3;
+  const bool live = item < a.n;
+  uint32_t s[8];
+  uint64_t off[kAdvanceWords] = {};
+  if (live) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) s[j] = a.in[8 * item + j];
+    for (uint32_t w = 0; w < a.words; ++w) off[w] = a.offsets[item * a.words + w];
+  } else {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) s[j] = 1u;
+  }
+  const int npos = 8 * (int)a.words;
+  Coef c = load_coef11(wide_rec(a, 0, wide_digit(off, 0), row));
+  for (int k = 0; k < npos; ++k) {
+    const bool last = k + 1 == npos;
+    const Coef next = load_coef11(wide_rec(a, last ? 0 : k + 1, last ? 0u : wide_digit(off, k + 1), row));
+    if (__ballot(wide_digit(off, k) != 0u)) jump_step(s, c, grp);
+    c = next;
+  }
+  if (live && row == 0) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) a.out[8 * item + j] = s[j];
+  }
+}
+
 template <int FMT, int PL>
 hipError_t fill_one(const FillArgs& a, int block_threads, hipStream_t stream) {
   const uint64_t waves = (a.n + a.tile - 1) / a.tile;
@@ -604,6 +659,15 @@ hipError_t launch_fill(const FillArgs& a, int format, int per_lane, int block_th
 
 hipError_t launch_batch(const BatchArgs& a, int grid, int block_threads, hipStream_t stream) {
   hipLaunchKernelGGL(k_batch, dim3(grid), dim3(block_threads), 0, stream, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_advance(const AdvanceArgs& a, hipStream_t stream) {
+  if (a.n == 0) return hipSuccess;
+  const uint64_t threads = a.n * 8;
+  if (a.n > (0xffffffffull >> 3) || a.words < 1 || a.words > (uint32_t)kAdvanceWords) return hipErrorInvalidValue;
+  const uint64_t blocks = (threads + 255) / 256;
+  hipLaunchKernelGGL(k_advance, dim3((uint32_t)blocks), dim3(256), 0, stream, a);
   return hipGetLastError();
 }
 

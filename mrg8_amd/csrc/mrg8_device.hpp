@@ -17,6 +17,11 @@ constexpr int kJumpPositions = 64 / kJumpBits;
 constexpr int kJumpValues = (1 << kJumpBits) - 1;  // non-zero digit values
 // Record index of row 0 of the identity, stored after the digit matrices.
 constexpr int kIdentityRec = kJumpPositions * kJumpValues * 8;
+// Bulk advances take offsets of up to kAdvanceWords 64-bit words; byte
+// positions kJumpPositions .. kWidePositions-1 live in a second table built
+// on first use.
+constexpr int kAdvanceWords = 4;
+constexpr int kWidePositions = 8 * kAdvanceWords;
 
 // Per-device constant tables.
 struct DeviceTables {
@@ -25,6 +30,11 @@ struct DeviceTables {
   // at ((k * kJumpValues) + (d - 1)) * 8 + r; then the 8 rows of the
   // identity at kIdentityRec + r
   CoefRecord* jump = nullptr;
+  // (kWidePositions - kJumpPositions) * kJumpValues * 8 records: row r of
+  // A^(d 2^(kJumpBits k)) for k >= kJumpPositions at
+  // (((k - kJumpPositions) * kJumpValues) + (d - 1)) * 8 + r; null until the
+  // first bulk advance with offsets wider than 64 bits
+  CoefRecord* wide = nullptr;
 };
 
 // Wave-uniform starting point of a launch.
@@ -107,10 +117,24 @@ struct BatchArgs {
   const CoefRecord* jump;
 };
 
+// A bulk advance: out[8i..8i+7] = in[8i..8i+7] jumped ahead by the
+// little-endian `words`-word offset offsets[words i .. words i + words - 1].
+struct AdvanceArgs {
+  const uint32_t* in;
+  const uint64_t* offsets;
+  uint32_t* out;
+  uint64_t n;
+  uint32_t words;  // 1..kAdvanceWords
+  const CoefRecord* jump;
+  const CoefRecord* wide;  // needed when words > 1
+};
+
 // Launchers (mrg8_device.hip).  All are asynchronous on `stream`.
 hipError_t launch_fill(const FillArgs& a, int format, int per_lane, int block_threads, hipStream_t stream);
 hipError_t launch_batch(const BatchArgs& a, int grid, int block_threads, hipStream_t stream);
 // One tile per wave: ceil(ntiles / waves per block) blocks.
 hipError_t launch_batch_tiles(const BatchArgs& a, uint64_t ntiles, int block_threads, hipStream_t stream);
+// Eight lanes per state; a.n * 8 must stay below 2^32.
+hipError_t launch_advance(const AdvanceArgs& a, hipStream_t stream);
 
 }  // namespace mrg8

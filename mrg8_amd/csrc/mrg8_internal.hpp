@@ -25,7 +25,12 @@ struct DeviceContext {
   DeviceTables tables;
   TunedShape tuned[3][64];  // [format][floor(log2 n)], set by mrg8_tune_fill
   std::mutex tuned_mu;      // guards `tuned` (fills on other handles read it)
+  std::mutex wide_mu;       // guards the first-use build of tables.wide
 };
+
+// Builds the table of byte positions kJumpPositions..kWidePositions-1 on the
+// context's device if it is not there yet.
+int ensure_wide_table(DeviceContext& ctx);
 
 // Records `fmt` as this thread's last error message and returns `code`.
 int fail(int code, const char* fmt, ...);

@@ -105,6 +105,35 @@ def test_jump_matrix_and_file(tmp_path, oracle_mod):
     p = tmp_path / "jump_matrix.txt"
     mrg8_amd.write_jump_matrix(str(p))
     assert hashlib.md5(p.read_bytes()).hexdigest() == GOLD["jump_matrix_md5"]
+    # read_jump_matrix (original/mrg8.cpp:88-102) gives back the same words
+    assert np.array_equal(mrg8_amd.read_jump_matrix(str(p)), oracle_mod.jump_table())
+    mrg8_amd.check_jump_matrix(str(p))
+
+
+def test_jump_matrix_file_errors(tmp_path, lib):
+    import mrg8_amd
+
+    good = tmp_path / "good.txt"
+    mrg8_amd.write_jump_matrix(str(good))
+    words = good.read_text().split()
+    bad = tmp_path / "bad.txt"  # one word changed (still a valid number)
+    w = words.copy()
+    w[64 * 100 + 9] = str((int(w[64 * 100 + 9]) + 1) % (2**31 - 1))
+    bad.write_text(" ".join(w))
+    with pytest.raises(lib.Mrg8Error) as e:
+        mrg8_amd.check_jump_matrix(str(bad))
+    assert e.value.code == lib.ERR_STATE and b"word 6409" in lib.lib().mrg8_last_error()
+    short = tmp_path / "short.txt"
+    short.write_text(" ".join(words[:-1]))
+    for path in (short, tmp_path / "missing.txt"):
+        with pytest.raises(lib.Mrg8Error) as e:
+            mrg8_amd.read_jump_matrix(str(path))
+        assert e.value.code == lib.ERR_IO
+    over = tmp_path / "over.txt"
+    over.write_text(" ".join(["2147483648"] + words[1:]))
+    with pytest.raises(lib.Mrg8Error) as e:
+        mrg8_amd.read_jump_matrix(str(over))
+    assert e.value.code == lib.ERR_IO
 
 
 def test_argument_errors(lib):
@@ -121,6 +150,10 @@ def test_argument_errors(lib):
     assert L.mrg8_jump_matrix(None, 0) == lib.ERR_ARG
     buf = (ctypes.c_uint32 * 10)()
     assert L.mrg8_jump_matrix(buf, 10) == lib.ERR_ARG
+    # bulk device jumps: argument checks come before any device work
+    assert L.mrg8_advance_states(0, None, None, 1, None, 5, None) == lib.ERR_ARG
+    assert L.mrg8_advance_states(0, 8, 8, 0, 8, 5, None) == lib.ERR_ARG  # words 0
+    assert L.mrg8_advance_states(0, None, None, 1, None, 0, None) == lib.OK  # nothing to do
     # null handles are errors, never crashes
     assert L.mrg8_batch_use_graph(None, 1) == lib.ERR_ARG
     assert L.mrg8_batch_run(None, None) == lib.ERR_ARG
