@@ -16,6 +16,8 @@ The same JSON line carries, measured in the same run:
   batched       4096 streams, log-uniform lengths in [2^10, 2^22), float64, one
                 launch with the autotuned shape
   jumps         65536 streams jumped by uniform 63-bit steps, 256 uint32 each
+  advance       2^20 device-resident states jumped by 200-bit (and 64-bit)
+                offsets (mrg8_advance_states)
   served        1e5 Poisson requests on 16384 resident streams, sizes
                 log-uniform in [2^8, 2^26) float64, 64 GB admission cap:
                 throughput and p50/p99 latency (arrival to completion)
@@ -210,6 +212,37 @@ def bench_jumps(args):
             "jump_macs_per_s": n * pc * 64 / (km * 1e-3)}
 
 
+def bench_advance(args, n=1 << 20):
+    """mrg8_advance_states: n device-resident states, each jumped by its own
+    random 200-bit offset (the reference's widest jump, done one state at a
+    time on the host there), and by 64-bit offsets for comparison."""
+    import mrg8_amd
+    from mrg8_amd import hip
+
+    rng = np.random.default_rng(7)
+    states = rng.integers(1, 2**31 - 1, size=(n, 8), dtype=np.int64).astype(np.uint32)
+    d_in, d_out = hip.DeviceBuffer(n * 32), hip.DeviceBuffer(n * 32)
+    d_in.copy_from_host(states)
+    out = {"states": n}
+    s = hip.Stream()
+    for words, bits in ((4, 200), (1, 64)):
+        offs = rng.integers(0, 2**63, size=(n, words), dtype=np.uint64) * np.uint64(2) + np.uint64(1)
+        if bits == 200:
+            offs[:, 3] &= np.uint64(0xFF)  # bits 200..255 clear
+        d_off = hip.DeviceBuffer(offs.nbytes)
+        d_off.copy_from_host(offs)
+        for _ in range(3):
+            mrg8_amd.advance_states(d_in, d_off, words, d_out, n, stream=s)
+        km = time_launches(lambda: mrg8_amd.advance_states(d_in, d_off, words, d_out, n, stream=s), s, 20)
+        d_off.free()
+        out[f"{bits}bit_ms"] = km
+        out[f"{bits}bit_states_per_s"] = n / (km * 1e-3)
+        log(f"advance: {n} states by {bits}-bit offsets in {km:.3f} ms ({n / (km * 1e-3) / 1e9:.2f} G states/s)")
+    d_in.free()
+    d_out.free()
+    return out
+
+
 def bench_served(args, capacity_gsps):
     import mrg8_amd
     from mrg8_amd import workloads
@@ -268,6 +301,18 @@ def cpu_baselines(args):
         mean = float(np.mean(ts[1:]))
         res["cpu_baseline"] = {"value": n / mean / 1e9, "unit": "G samples/s", "cores": 1, "kind": "port",
                                "sample": f"oracle serial rand, 2^{args.cpu_log2} outputs, mean of 3"}
+    # the reference's 200-bit jump (J = product of table matrices, then J x state), one state at a time
+    rng = np.random.default_rng(7)
+    nj = 2000
+    offs = [int.from_bytes(rng.bytes(25), "little") for _ in range(nj)]
+    st = oracle.seed_state(SEED)
+    t0 = time.perf_counter()
+    for x in offs:
+        oracle.jump_bits(st, [b for b in range(200) if (x >> b) & 1])
+    dt = time.perf_counter() - t0
+    res["cpu_baseline_advance"] = {"value": nj / dt, "unit": "states/s", "cores": 1, "kind": "port",
+                                   "sample": f"oracle restatement of original/mrg8.cpp:105-122 (jump_ahead(short[200])), "
+                                             f"{nj} random 200-bit offsets, one pass"}
     # vector + OpenMP restatement of the knl path on every host core
     nv = 1 << 28
     buf = np.empty(nv, dtype=np.float64)
@@ -364,6 +409,7 @@ def main():
     if not args.quick:
         line["batched"] = bench_batched(args)
         line["jumps"] = bench_jumps(args)
+        line["advance"] = bench_advance(args)
         line["served"] = bench_served(args, line["batched"]["gsps"])
         line["host_form_f64"] = bench_host_form()
     if not args.quick and not args.no_comparators:
