@@ -341,3 +341,28 @@ def test_mutations_are_caught(oracle_mod):
         x = (x * 6364136223846793007) % 2**64
         wrong.append(x >> 33)
     assert wrong != GOLD["seed_states"][str(SEED)]
+
+
+def test_wide_jump_matrices_have_no_zero_entries(oracle_mod):
+    """A^(d 256^k) for byte positions k = 8..31 (the device's table for
+    offsets past 64 bits) has no entry equal to 0 mod p, so the exact-zero
+    case of bigDotProd (original/mrg8.cpp:72-82) depends only on the state's
+    zero words there, as the device's support masks assume."""
+    p = 2**31 - 1
+    c = [int(x) for x in oracle_mod.coeffs()]
+    a = [c] + [[1 if j == r - 1 else 0 for j in range(8)] for r in range(1, 8)]
+
+    def mul(x, y):
+        return [[sum(x[r][k] * y[k][j] for k in range(8)) % p for j in range(8)] for r in range(8)]
+
+    m = a
+    for _ in range(64):
+        m = mul(m, m)  # A^(2^64)
+    for _k in range(8, 32):
+        q = m
+        for d in range(1, 256):
+            if d > 1:
+                q = mul(q, m)
+            assert all(v != 0 for row in q for v in row)
+        for _ in range(8):
+            m = mul(m, m)
