@@ -366,3 +366,17 @@ def test_wide_jump_matrices_have_no_zero_entries(oracle_mod):
             assert all(v != 0 for row in q for v in row)
         for _ in range(8):
             m = mul(m, m)
+
+
+@pytest.mark.parametrize("n", [1, 7, 8, 9, 1000, 65537, (1 << 20) + 3])
+@pytest.mark.parametrize("threads", [1, 3, 8])
+def test_vector_baseline_equals_serial(oracle_mod, n, threads):
+    """The vector + OpenMP restatement of knl/mrg8_vec.h (the timed CPU
+    baseline) gives the serial generator's [0,1) doubles bit for bit: the
+    reference's only bit-exact check, knl/tests/test_mrg8_vec_outer.cpp:70-83
+    (seed 13579, mrg8_vec_outer against mrg8::rand), asserted here."""
+    st = oracle_mod.seed_state(13579)
+    got, _, used = oracle_mod.vec_outer_tp(st, n, threads)
+    want, _ = oracle_mod.generate(st, n, 1)
+    assert used >= 1
+    assert np.array_equal(got.view(np.uint64), want.view(np.uint64))
