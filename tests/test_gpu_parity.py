@@ -418,3 +418,22 @@ def test_c2_statistics_like_check_rand(gpu, fmt):
         assert lo > 0.0 and hi <= 1.0
     else:
         assert lo >= 0.0 and hi < 1.0
+
+
+def test_benchmark_driver_harness(gpu, capsys):
+    """The counterpart of cuda/tests/test_mrg8_outer.cu / _inner.cu prints the
+    reference's EVALUATION lines for a short sweep."""
+    from mrg8_amd import harness
+
+    say("harness")
+    assert harness.main(["1", "--tnum-max", "2048", "--auto"]) == 0
+    assert harness.main(["1", "--tnum-max", "1024", "--inner"]) == 0
+    out = capsys.readouterr().out
+    lines = [l for l in out.splitlines() if l.startswith("EVALUATION")]
+    # outer: TNUM 1024 x BS 32..1024 (6) + TNUM 2048 (6) + auto; inner: TNUM 1024
+    assert len(lines) == 6 + 6 + 1 + 1, lines
+    f = lines[0].split(", ")
+    assert f[1] == "MRG8_HIP_outer" and f[2] == "1024" and f[3] == "32" and f[4] == str(1 << 20)
+    assert float(f[5]) > 0 and float(f[6]) > 0
+    assert lines[-1].split(", ")[3] == ""  # inner: empty BS field
+    assert "Arithmetic mean: 0.49" in out or "Arithmetic mean: 0.50" in out
