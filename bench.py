@@ -344,6 +344,19 @@ def cpu_baselines(args):
                                              "oracle serial recurrence per stream, OpenMP over streams, "
                                              "outputs checksummed in 4096-value blocks instead of stored; one run"}
     log(f"cpu batched: {res['cpu_baseline_batched']['value']:.2f} G samples/s on {threads} threads")
+    # the jump-heavy batch: 65536 x (63-bit jump by the knl matrix-vector chain
+    # + 256 words), OpenMP over streams
+    steps = workloads.jump_steps()
+    m = len(steps)
+    st = np.tile(oracle.seed_state(SEED), (m, 1))
+    t0 = time.perf_counter()
+    oracle.stream_wsums(st, steps, np.full(m, 256, dtype=np.uint64), np.full(m, 2, dtype=np.int32))
+    wall = time.perf_counter() - t0
+    res["cpu_baseline_jumps"] = {"value": wall * 1e3, "unit": "ms", "cores": threads, "kind": "port",
+                                 "sample": "the whole jump-heavy batch (65536 streams: jump by the matrix-vector chain "
+                                           "of knl/mrg8.h:212-238, then 256 words each, checksummed), OpenMP over "
+                                           "streams; one run"}
+    log(f"cpu jumps: {wall * 1e3:.1f} ms on {threads} threads")
     try:
         res["host_cpu"] = subprocess.run(["sh", "-c", "grep -m1 'model name' /proc/cpuinfo | cut -d: -f2"],
                                          capture_output=True, text=True).stdout.strip()
