@@ -357,6 +357,23 @@ def cpu_baselines(args):
                                            "of knl/mrg8.h:212-238, then 256 words each, checksummed), OpenMP over "
                                            "streams; one run"}
     log(f"cpu jumps: {wall * 1e3:.1f} ms on {threads} threads")
+    # the served mix: the first 5000 requests (about 2.7e10 float64), each a
+    # jump to its reserved position + its outputs, OpenMP over requests
+    states, _, ids, sizes = workloads.served()
+    k = 5000
+    cursor = np.zeros(len(states), dtype=np.uint64)
+    offs = np.empty(k, dtype=np.uint64)
+    for i in range(k):
+        offs[i] = cursor[ids[i]]
+        cursor[ids[i]] += sizes[i]
+    t0 = time.perf_counter()
+    oracle.stream_wsums(states[ids[:k]], offs, sizes[:k], np.zeros(k, dtype=np.int32))
+    wall = time.perf_counter() - t0
+    res["cpu_baseline_served"] = {"value": float(sizes[:k].sum()) / wall / 1e9, "unit": "G samples/s",
+                                  "cores": threads, "kind": "port",
+                                  "sample": f"first {k} requests of the served mix ({float(sizes[:k].sum()):.3g} "
+                                            "float64), throughput only (no arrivals), OpenMP over requests; one run"}
+    log(f"cpu served: {res['cpu_baseline_served']['value']:.2f} G samples/s on {threads} threads")
     try:
         res["host_cpu"] = subprocess.run(["sh", "-c", "grep -m1 'model name' /proc/cpuinfo | cut -d: -f2"],
                                          capture_output=True, text=True).stdout.strip()
