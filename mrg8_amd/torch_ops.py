@@ -11,6 +11,10 @@
   ``out``, on the current stream.  ``offset`` is a schema ``int`` (signed
   64-bit), so it reaches 2^63 - 1; larger offsets go through ``Generator``.
 
+* ``torch.ops.mrg8.advance_(states, offsets)`` jumps rows of an (n, 8) state
+  tensor by up to 256-bit offsets on the device; ``substream_states(seed, n)``
+  builds n substreams spaced 2^100 apart with it.
+
 Formats: ``F64`` (x / p in (0, 1], original/mrg8.cpp:233-236) and ``F64_01``
 ((x - 1) / p in [0, 1), cuda/mrg8_cuda.cu:363) fill float64 tensors; ``U32``
 (the raw word x in [1, p]) fills int32 or uint32 tensors (p = 2^31 - 1 fits
@@ -74,6 +78,54 @@ def _(out, state, offset, fmt):
     return None
 
 
+@torch.library.custom_op("mrg8::advance_", mutates_args=("states",))
+def advance_(states: torch.Tensor, offsets: torch.Tensor) -> None:
+    """Jumps each row of `states` ((n, 8) int32/uint32 words, newest first) ahead
+    by the matching row of `offsets` ((n, words) int64/uint64, little-endian
+    64-bit words, words 1..4: up to 256-bit offsets), in place, on the current
+    stream (the bulk device jump of mrg8_advance_states)."""
+    if states.device.type != "cuda" or offsets.device != states.device:
+        raise ValueError("states and offsets must be HIP tensors on one device")
+    if states.dtype not in (torch.int32, torch.uint32) or states.dim() != 2 or states.shape[1] != 8:
+        raise TypeError("states must be an (n, 8) int32 or uint32 tensor")
+    if offsets.dtype not in (torch.int64, torch.uint64) or offsets.dim() != 2 or offsets.shape[0] != states.shape[0]:
+        raise TypeError("offsets must be an (n, words) int64 or uint64 tensor")
+    if not (states.is_contiguous() and offsets.is_contiguous()):
+        raise ValueError("states and offsets must be contiguous")
+    n, words = states.shape[0], offsets.shape[1]
+    if n == 0:
+        return
+    from . import advance_states
+
+    advance_states(states, offsets, words, states, n, stream=torch.cuda.current_stream(states.device),
+                   device=states.device.index if states.device.index is not None else torch.cuda.current_device())
+
+
+@advance_.register_fake
+def _(states, offsets):
+    return None
+
+
+def substream_states(seed: int, n: int, log2_spacing: int = 100, device=None) -> torch.Tensor:
+    """(n, 8) int32 states: row i is the stream of `seed` (seeded like the
+    reference's mrg8(seed)) advanced by i * 2^log2_spacing positions, computed
+    on the device -- n independent substreams for n workers."""
+    from . import seed_state
+
+    device = torch.device(device if device is not None else "cuda")
+    if n < 0 or log2_spacing < 0 or max(n - 1, 0).bit_length() + log2_spacing > 256:
+        raise ValueError("need n >= 0 and n * 2^log2_spacing below 2^256")
+    if n == 0:
+        return torch.empty((0, 8), dtype=torch.int32, device=device)
+    words = max(1, ((max(n - 1, 1)).bit_length() + log2_spacing + 63) // 64)
+    base = torch.tensor(seed_state(seed).astype("int64"), dtype=torch.int64).to(torch.int32)
+    states = base.repeat(n, 1).to(device)
+    offs = [[((i << log2_spacing) >> (64 * w)) & (2**64 - 1) for w in range(words)] for i in range(n)]
+    offsets = torch.tensor(offs, dtype=torch.uint64).to(device)
+    torch.ops.mrg8.advance_(states, offsets)
+    return states
+
+
 class Generator:
     """One MRG8 stream producing torch tensors (see the module docstring)."""
 
@@ -119,4 +171,4 @@ class Generator:
         return self._fill(torch.empty(*size, dtype=dtype, device=self.device), U32)
 
 
-__all__ = ["Generator", "fill_", "F64", "F64_01", "U32"]
+__all__ = ["Generator", "fill_", "advance_", "substream_states", "F64", "F64_01", "U32"]

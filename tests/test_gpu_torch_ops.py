@@ -57,3 +57,33 @@ def test_fill_op_on_side_stream(gpu, oracle_mod):
         torch.ops.mrg8.fill_(torch.empty(4, dtype=torch.float32, device="cuda:0"), state, 0, torch_ops.F64)
     with pytest.raises(ValueError):
         torch.ops.mrg8.fill_(out[:, ::2], state, 0, torch_ops.F64)
+
+
+def test_advance_op_and_substreams(gpu, oracle_mod):
+    import torch
+
+    from mrg8_amd import torch_ops
+
+    rng = np.random.default_rng(8)
+    n = 37
+    st = rng.integers(1, 2**31 - 1, size=(n, 8)).astype(np.int32)
+    offs = rng.integers(0, 2**63, size=(n, 3), dtype=np.int64)
+    offs[:, 2] &= 0xFF  # offsets below 2^136
+    states = torch.from_numpy(st).to("cuda:0")
+    offsets = torch.from_numpy(offs).to("cuda:0")
+    side = torch.cuda.Stream()
+    say("torch-advance-op")
+    with torch.cuda.stream(side):
+        torch.ops.mrg8.advance_(states, offsets)
+    side.synchronize()
+    got = states.cpu().numpy().view(np.uint32)
+    for i in range(n):
+        x = sum(int(np.uint64(offs[i, w])) << (64 * w) for w in range(3))
+        want = oracle_mod.jump_bits(st[i].view(np.uint32), [b for b in range(x.bit_length()) if (x >> b) & 1])
+        assert got[i].tolist() == want.tolist(), i
+    sub = torch_ops.substream_states(13579, 16, 100, device="cuda:0").cpu().numpy().view(np.uint32)
+    s0 = oracle_mod.seed_state(13579)
+    for i in range(16):
+        want = oracle_mod.jump_bits(s0, [100 + b for b in range(i.bit_length()) if (i >> b) & 1])
+        assert sub[i].tolist() == want.tolist(), i
+    assert torch_ops.substream_states(1, 0, device="cuda:0").shape == (0, 8)

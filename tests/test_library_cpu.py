@@ -233,3 +233,6 @@ def test_torch_op_registered_and_rejects_cpu_tensors():
         torch.ops.mrg8.fill_(torch.empty(4, dtype=torch.float64), [1] * 8, 0, torch_ops.F64)
     with pytest.raises(ValueError):
         torch_ops.Generator(1, device="cpu")
+    assert hasattr(torch.ops.mrg8, "advance_")
+    with pytest.raises(ValueError):
+        torch.ops.mrg8.advance_(torch.ones((2, 8), dtype=torch.int32), torch.zeros((2, 1), dtype=torch.int64))
