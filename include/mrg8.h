@@ -225,7 +225,10 @@ typedef struct mrg8_batch_config {
 
 /* A batch object owns device descriptor and tile tables sized for
  * `max_streams` segments and `max_tiles` work items (no reference
- * counterpart: the reference generates one stream per blocking call). */
+ * counterpart: the reference generates one stream per blocking call).  A
+ * segment of c outputs takes at most ceil(c / tile_outputs) tiles: its first
+ * tile also takes the outputs before its first tile-aligned address (a power
+ * of two dividing tile_outputs times the element size). */
 int mrg8_batch_create(int device, uint32_t max_streams, uint64_t max_tiles, mrg8_batch_t* out);
 int mrg8_batch_destroy(mrg8_batch_t b);
 int mrg8_batch_get_config(mrg8_batch_t b, mrg8_batch_config* cfg);

@@ -471,9 +471,9 @@ __device__ __forceinline__ TileInfo lookup_tile(const BatchTables& tab, const Ba
   TileInfo ti;
   ti.fmt = sp->format;
   const uint64_t kt = t - tab.tile_begin[seg];  // tile number within the segment
-  const uint64_t head = segment_head(sp->out, ti.fmt, sp->count);
-  const uint64_t first = kt ? head + kt * a.tile : 0;
-  ti.cnt = min(sp->count, head + (kt + 1) * a.tile) - first;
+  uint64_t first, end;
+  tile_bounds(segment_head(sp->out, ti.fmt, sp->count, a.tile), sp->count, a.tile, kt, &first, &end);
+  ti.cnt = end - first;
   ti.dst = reinterpret_cast<uint64_t>(sp->out) + first * (ti.fmt == 2 ? 4u : 8u);
   ti.offset = sp->offset + first;
   const uint32_t sid = sp->stream;

@@ -315,6 +315,7 @@ extern "C" int mrg8_server_create(int device, const mrg8_server_config* cfg, uin
   s->arena.init(s->cfg.max_output_bytes);
   // Segments per step: every admitted request may contribute one chunk.
   const uint32_t seg_cap = std::min<uint32_t>(s->cfg.max_inflight, 1u << 16);
+  // a chunk of c outputs takes at most ceil(c / tile) tiles
   const uint64_t tile_cap = s->cfg.step_outputs / s->cfg.tile_outputs + seg_cap + 1;
   for (uint32_t i = 0; i < s->cfg.steps_in_flight; ++i) {
     auto st = std::make_unique<Step>();

@@ -108,33 +108,36 @@ def test_batch_restaged_on_another_stream(gpu, oracle_mod, graph):
 
 @pytest.mark.parametrize("bpc", [1, 0], ids=["persistent", "tile-per-wave"])
 def test_batch_tile_heads(gpu, oracle_mod, bpc):
-    """Segments at every element offset within a 128-byte line, with counts
-    around the head (the outputs before the first 128-byte boundary, which
-    tile 0 of a segment also writes) and the tile size: outputs match the
-    oracle and the bytes around every segment stay untouched."""
+    """Segments at many element offsets within a tile-alignment unit (the
+    largest power of two dividing tile * element size), with counts around the
+    head (the outputs before the first such boundary, which tile 0 of a
+    segment also writes) and the tile size: outputs match the oracle and the
+    bytes around every segment stay untouched."""
     from mrg8_amd.hip import DeviceBuffer
 
     tile = 256
     cases = []
     for fmt, esz in ((0, 8), (2, 4), (1, 8)):
-        for m in range(0, 128, esz):
-            head = ((128 - m) % 128) // esz
+        align = tile * esz  # a power of two here
+        for m in sorted(set(range(0, 128, esz)) | {align // 2, align // 2 + esz, align - 2 * esz, align - esz}):
+            head = ((align - m) % align) // esz
             for c in sorted({1, max(1, head - 1), max(1, head), head + 1, head + tile, head + tile + 1,
                              head + 3 * tile + 5}):
                 cases.append((fmt, esz, m, c))
-    slot = (max(c * esz for _, esz, _, c in cases) + 256) // 128 * 128 + 128
+    slot = (max(m + c * esz for _, esz, m, c in cases) + 256 + 2047) // 2048 * 2048
     rng = np.random.default_rng(77)
     states = rng.integers(1, 2**31 - 1, size=(len(cases), 8)).astype(np.uint32)
     offs = [int(x) for x in rng.integers(0, 2**63, size=len(cases), dtype=np.uint64)]
-    buf = DeviceBuffer(slot * len(cases))
+    buf = DeviceBuffer(slot * len(cases) + 2048)
     buf.fill_byte(0xAB)
+    base = (buf.ptr + 2047) // 2048 * 2048  # every slot starts on a 2 KiB boundary
     b = gpu.Batch(max_streams=len(cases), max_tiles=1 << 16)
     b.set_config(256, bpc, tile)  # persistent: a small grid, every wave takes several tiles
-    b.set([(states[i], offs[i], c, buf.ptr + i * slot + m, fmt) for i, (fmt, _, m, c) in enumerate(cases)])
+    b.set([(states[i], offs[i], c, base + i * slot + m, fmt) for i, (fmt, _, m, c) in enumerate(cases)])
     say("batch-tile-heads", len(cases))
     b.run()
     gpu.hip.synchronize()
-    host = buf.copy_to_host(np.uint8, slot * len(cases), 0)
+    host = buf.copy_to_host(np.uint8, slot * len(cases), base - buf.ptr)
     for i, (fmt, esz, m, c) in enumerate(cases):
         want, _ = oracle_mod.generate(oracle_mod.jump(states[i], offs[i]), c, fmt)
         region = host[i * slot:(i + 1) * slot]
