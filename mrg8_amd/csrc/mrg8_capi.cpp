@@ -176,15 +176,6 @@ int format_size(int format) {
   }
 }
 
-// Outputs per lane for a destination address: 16-byte stores when aligned
-// (words: one 16-byte store per lane; doubles: two, each part of a
-// contiguous 1 KiB store of the wave).
-int pick_per_lane(int format, const void* p) {
-  const uintptr_t a = reinterpret_cast<uintptr_t>(p);
-  if (format == MRG8_U32) return (a & 15u) == 0 ? 4 : ((a & 7u) == 0 ? 2 : 1);
-  return (a & 15u) == 0 ? 4 : 1;
-}
-
 // Default launch shape for a single-stream fill of n outputs.  Measured on
 // MI355X (2^28 outputs): doubles are fastest with 1024-thread blocks, 4
 // outputs per lane and 2048 outputs per wave, words with 256-thread blocks
@@ -203,31 +194,27 @@ int fill_device(DeviceContext& ctx, const uint32_t state[8], uint64_t offset, vo
                 hipStream_t stream, int block_threads, uint64_t tile, int per_lane) {
   if (n == 0) return MRG8_OK;
   const int esz = format_size(format);
-  // Outputs before the first 16-byte boundary are written by wave 0 of block 0
-  // (peel); the waves then cover the aligned remainder with wide stores.
-  uint32_t peel = (uint32_t)(((16u - (reinterpret_cast<uintptr_t>(dptr) & 15u)) & 15u) / (uint32_t)esz);
-  if (peel > n) peel = (uint32_t)n;
-  void* main_ptr = static_cast<char*>(dptr) + (size_t)peel * esz;
-  const uint64_t main_n = n - peel;
-  int pl = pick_per_lane(format, main_ptr);
+  // 4 outputs per lane (16-byte stores) whatever the destination's alignment:
+  // each wave writes the outputs before its first 1 KiB boundary one per lane.
+  int pl = 4;
   if (per_lane) {
     if (per_lane != 1 && per_lane != 2 && per_lane != 4) return fail(MRG8_ERR_ARG, "outputs_per_lane must be 1, 2 or 4");
     pl = per_lane;
   }
   int def_block = 0;
   uint64_t def_tile = 0;
-  const int cls = main_n ? 63 - __builtin_clzll(main_n) : 0;
+  const int cls = 63 - __builtin_clzll(n);
   TunedShape tuned;
   {
     std::lock_guard<std::mutex> lk(ctx.tuned_mu);
     tuned = ctx.tuned[format][cls];
   }
-  if (tuned.valid && !per_lane && (tuned.per_lane == 1 || (reinterpret_cast<uintptr_t>(main_ptr) & 15u) == 0)) {
+  if (tuned.valid && !per_lane) {
     pl = tuned.per_lane;
     def_block = tuned.block_threads;
     def_tile = tuned.tile;
   } else {
-    default_shape(ctx, main_n, format, pl, &def_block, &def_tile);
+    default_shape(ctx, n, format, pl, &def_block, &def_tile);
   }
   if (block_threads <= 0) block_threads = def_block;
   if (block_threads < 64 || block_threads > 1024 || block_threads % 64)
@@ -235,17 +222,24 @@ int fill_device(DeviceContext& ctx, const uint32_t state[8], uint64_t offset, vo
   const uint64_t quantum = 64ull * pl;
   if (tile == 0) tile = def_tile;
   if (tile % quantum) tile = (tile + quantum - 1) / quantum * quantum;
+  // Wave 0 also takes the outputs before the first 1 KiB boundary, so every
+  // other wave starts on one and writes whole cache lines (a 2^28-double fill
+  // 8 bytes off a boundary took 471 us with waves cut from the destination
+  // as given, 346-359 us so; a head up to the next 16 KiB boundary left wave
+  // 0 with up to two tiles of work and the fill 3 % slower).
+  const uint64_t align = 1024;
+  uint64_t head = ((align - (reinterpret_cast<uintptr_t>(dptr) & (align - 1))) & (align - 1)) / (uint64_t)esz;
+  if (head > n) head = n;
   // The grid (work-items in total) is a 32-bit quantity.
   const uint64_t wpb = (uint64_t)block_threads / 64;
-  if (((main_n + tile - 1) / tile + wpb - 1) / wpb * (uint64_t)block_threads > 0xffffffffull)
+  if ((fill_waves(n, head, tile) + wpb - 1) / wpb * (uint64_t)block_threads > 0xffffffffull)
     return fail(MRG8_ERR_ARG, "%llu outputs at %llu per wave need more than 2^32 - 1 work-items; use more outputs per wave",
-                (unsigned long long)main_n, (unsigned long long)tile);
+                (unsigned long long)n, (unsigned long long)tile);
   FillArgs a{};
-  a.out = main_ptr;
-  a.n = main_n;
+  a.out = dptr;
+  a.n = n;
   a.tile = tile;
-  a.peel = peel;
-  a.peel_out = dptr;
+  a.head = head;
   std::memcpy(a.start.state, state, sizeof a.start.state);
   a.start.offset = offset;
   a.lane = ctx.tables.lane;
@@ -491,8 +485,7 @@ static int drop_in_fill(mrg8_gen_t g, double* d, int n, int bs, int tnum) {
   if (n == 0) return MRG8_OK;
   // TNUM threads in waves of 64: each wave covers n / (TNUM / 64) outputs.
   const uint64_t waves = std::max<uint64_t>(1, (uint64_t)tnum / 64);
-  const int pl = pick_per_lane(MRG8_F64_01, d);
-  const uint64_t quantum = 64ull * pl;
+  const uint64_t quantum = 64ull * 4;  // 4 outputs per lane
   uint64_t tile = ((uint64_t)n + waves - 1) / waves;
   tile = std::max<uint64_t>(quantum, (tile + quantum - 1) / quantum * quantum);
   int block = bs < 64 ? 64 : (bs > 1024 ? 1024 : bs / 64 * 64);

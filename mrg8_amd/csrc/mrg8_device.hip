@@ -287,53 +287,8 @@ __device__ __forceinline__ void load_lane_coefs(Coef (&k)[PL], const CoefRecord*
   for (int i = 0; i < PL; ++i) k[i] = load_coef(lane_rec + LaneLayout<FMT, PL>::index(lane, i));
 }
 
-// Generates `cnt` outputs of the stream whose wave-uniform state (at the
-// first output) is s, into dst[0..cnt), with this lane's records k.  `cnt`
-// and s must be wave-uniform.
-template <int FMT, int PL>
-__device__ __forceinline__ void generate_run(typename Out<FMT>::T* __restrict__ dst, uint64_t cnt, uint32_t s[8],
-                                             const Coef (&k)[PL]) {
-  typedef LaneLayout<FMT, PL> L;
-  constexpr int kPer = 64 * PL;
-  const uint32_t lane = threadIdx.x & 63;
-  typename Out<FMT>::T* base = dst;
-  const uint64_t full = cnt / kPer;
-  const uint64_t rem = cnt - full * kPer;
-  uint32_t any = 0;
-#pragma unroll
-  for (int j = 0; j < 8; ++j) any |= s[j];
-  if (any == 0u) {  // degenerate all-zero state
-    for (uint64_t i = lane; i < cnt; i += 64) __builtin_nontemporal_store(convert_zero<FMT>(), dst + i);
-    return;
-  }
-  for (uint64_t it = 0; it < full; ++it) {
-    int32_t e[8];
-    centre(s, e);
-    uint32_t r[PL];
-    dot_residues<PL>(k, e, r);
-    store_lane<FMT, PL>(base, lane, r);
-    base += kPer;
-    // next state: the last eight words, newest first
-#pragma unroll
-    for (int j = 0; j < 8; ++j)
-      s[j] = __builtin_amdgcn_readlane(r[L::slot_of(kPer - 1 - j)], L::lane_of(kPer - 1 - j)) + 1u;
-  }
-  if (rem) {
-    int32_t e[8];
-    centre(s, e);
-    uint32_t r[PL];
-    dot_residues<PL>(k, e, r);
-#pragma unroll
-    for (int i = 0; i < PL; ++i) {
-      const uint32_t o = L::index(lane, i);
-      if (o < rem) __builtin_nontemporal_store(convert<FMT>(r[i]), base + o);
-    }
-  }
-}
-
 // Writes the first k (< 64) outputs of a run one per lane and advances the
-// wave-uniform state past them: used to reach 16-byte alignment before the
-// wide-store loop.
+// wave-uniform state past them.
 template <int FMT>
 __device__ __forceinline__ void peel_outputs(typename Out<FMT>::T* __restrict__ dst, uint32_t k, uint32_t s[8],
                                              const CoefRecord* __restrict__ lane_rec) {
@@ -366,6 +321,79 @@ __device__ __forceinline__ void peel_outputs(typename Out<FMT>::T* __restrict__ 
   for (int j = 0; j < 8; ++j) s[j] = ns[j];
 }
 
+// Byte alignment of the wide-store loop's wave stores: 64 lanes x 16 bytes.
+constexpr uintptr_t kStoreAlign = 1024;
+
+// Writes the outputs of a run before its first kStoreAlign boundary (at most
+// 63 per round, one per lane) and advances dst, cnt and the state past them,
+// so that every 1 KiB wave store of the loop that follows covers whole cache
+// lines: on MI355X a 2^28-double fill whose destination was 8 or 16 bytes off
+// such a boundary took 471-473 us against 342-347 us.
+template <int FMT, uintptr_t ALIGN>
+__device__ __forceinline__ void peel_to_boundary(typename Out<FMT>::T*& dst, uint64_t& cnt, uint32_t s[8],
+                                                 const CoefRecord* __restrict__ lane_rec) {
+  typedef typename Out<FMT>::T T;
+  uint64_t b = ((ALIGN - (reinterpret_cast<uintptr_t>(dst) & (ALIGN - 1))) & (ALIGN - 1)) / sizeof(T);
+  if (b > cnt) b = cnt;
+  while (b) {
+    const uint32_t k = b < 63 ? (uint32_t)b : 63u;
+    peel_outputs<FMT>(dst, k, s, lane_rec);
+    dst += k;
+    cnt -= k;
+    b -= k;
+  }
+}
+
+// Generates `cnt` outputs of the stream whose wave-uniform state (at the
+// first output) is s, into dst[0..cnt), with this lane's records k (and the
+// lane records, for the outputs before the first 1 KiB boundary).  `cnt` and
+// s must be wave-uniform; dst need only be element-aligned.
+template <int FMT, int PL, uintptr_t ALIGN = kStoreAlign>
+__device__ __forceinline__ void generate_run(typename Out<FMT>::T* __restrict__ dst, uint64_t cnt, uint32_t s[8],
+                                             const Coef (&k)[PL], const CoefRecord* __restrict__ lane_rec) {
+  typedef LaneLayout<FMT, PL> L;
+  constexpr int kPer = 64 * PL;
+  const uint32_t lane = threadIdx.x & 63;
+  uint32_t any = 0;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) any |= s[j];
+  if (any == 0u) {  // degenerate all-zero state
+    for (uint64_t i = lane; i < cnt; i += 64) __builtin_nontemporal_store(convert_zero<FMT>(), dst + i);
+    return;
+  }
+  typename Out<FMT>::T* base = dst;
+  peel_to_boundary<FMT, ALIGN>(base, cnt, s, lane_rec);
+  const uint64_t full = cnt / kPer;
+  const uint64_t rem = cnt - full * kPer;
+  for (uint64_t it = 0; it < full; ++it) {
+    int32_t e[8];
+    centre(s, e);
+    uint32_t r[PL];
+    dot_residues<PL>(k, e, r);
+    store_lane<FMT, PL>(base, lane, r);
+    base += kPer;
+    // next state: the last eight words, newest first
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+      s[j] = __builtin_amdgcn_readlane(r[L::slot_of(kPer - 1 - j)], L::lane_of(kPer - 1 - j)) + 1u;
+  }
+  if (rem) {
+    int32_t e[8];
+    centre(s, e);
+    uint32_t r[PL];
+    dot_residues<PL>(k, e, r);
+#pragma unroll
+    for (int i = 0; i < PL; ++i) {
+      const uint32_t o = L::index(lane, i);
+      if (o < rem) __builtin_nontemporal_store(convert<FMT>(r[i]), base + o);
+    }
+  }
+}
+
+// First output of wave w of a fill: wave 0 also takes the head (the outputs
+// before the first tile-aligned address), every later wave starts on one.
+__device__ __forceinline__ uint64_t fill_first(const FillArgs& a, uint64_t w) { return w ? a.head + w * a.tile : 0; }
+
 template <int FMT, int PL>
 __global__ __launch_bounds__(1024) void k_fill(FillArgs a) {
   // Start states of the block's waves: wave j (j < ceil(nw / 8)) jumps the
@@ -381,55 +409,31 @@ __global__ __launch_bounds__(1024) void k_fill(FillArgs a) {
     uint32_t s[8];
 #pragma unroll
     for (int j = 0; j < 8; ++j) s[j] = a.start.state[j];
-    jump_groups(s, target < nw ? a.start.offset + a.peel + (wave0 + target) * a.tile : 0, a.jump);
+    jump_groups(s, target < nw ? a.start.offset + fill_first(a, wave0 + target) : 0, a.jump);
     if (target < nw && (lane & 7) == 0) {
 #pragma unroll
       for (int j = 0; j < 8; ++j) start_state[target][j] = s[j];
     }
   }
   __syncthreads();
-  if (a.peel && blockIdx.x == 0 && wib == 0) {  // the outputs before the first 16-byte boundary
-    uint32_t s[8];
-#pragma unroll
-    for (int j = 0; j < 8; ++j) s[j] = a.start.state[j];
-    jump_groups(s, lane < 8 ? a.start.offset : 0, a.jump);
-#pragma unroll
-    for (int j = 0; j < 8; ++j) s[j] = __builtin_amdgcn_readfirstlane(s[j]);
-    peel_outputs<FMT>(static_cast<typename Out<FMT>::T*>(a.peel_out), a.peel, s, a.lane);
-  }
-  const uint64_t first = (wave0 + wib) * a.tile;
+  const uint64_t w = wave0 + wib;
+  const uint64_t first = fill_first(a, w);
   if (first >= a.n) return;
-  const uint64_t cnt = min(a.tile, a.n - first);
+  const uint64_t cnt = min(a.head + (w + 1) * a.tile, a.n) - first;
   uint32_t s[8];
 #pragma unroll
   for (int j = 0; j < 8; ++j) s[j] = __builtin_amdgcn_readfirstlane(start_state[wib][j]);
   Coef k[PL];
   load_lane_coefs<FMT, PL>(k, a.lane);
-  generate_run<FMT, PL>(static_cast<typename Out<FMT>::T*>(a.out) + first, cnt, s, k);
+  generate_run<FMT, PL>(static_cast<typename Out<FMT>::T*>(a.out) + first, cnt, s, k, a.lane);
 }
 
-// Outputs needed before `p` is 16-byte aligned (p itself element-aligned).
-template <typename T>
-__device__ __forceinline__ uint32_t peel_count(const T* p) {
-  return (uint32_t)(((16u - (reinterpret_cast<uintptr_t>(p) & 15u)) & 15u) / sizeof(T));
-}
-
-// One tile of a batch segment at dst: peels up to one double (three words)
-// to reach 16-byte alignment (only tile 0 of a segment can need it), then
-// runs the wide-store loop.
-template <int FMT, int PL>
+// One tile of a batch segment at dst (generate_run peels the outputs before
+// the first 1 KiB boundary; only tile 0 of a segment can have any).
+template <int FMT, int PL, uintptr_t ALIGN = kStoreAlign>
 __device__ __forceinline__ void batch_tile(void* out, uint64_t cnt, uint32_t s[8], const Coef (&k)[PL],
                                            const CoefRecord* __restrict__ lane_rec) {
-  typedef typename Out<FMT>::T T;
-  T* dst = static_cast<T*>(out);
-  uint32_t peel = peel_count(dst);
-  if (peel > cnt) peel = (uint32_t)cnt;
-  if (peel) {
-    peel_outputs<FMT>(dst, peel, s, lane_rec);
-    dst += peel;
-    cnt -= peel;
-  }
-  if (cnt) generate_run<FMT, PL>(dst, cnt, s, k);
+  if (cnt) generate_run<FMT, PL, ALIGN>(static_cast<typename Out<FMT>::T*>(out), cnt, s, k, lane_rec);
 }
 
 __device__ __forceinline__ uint64_t readlane_u64(uint64_t v, int lane) {
@@ -517,9 +521,9 @@ __global__ __launch_bounds__(1024) void k_batch(BatchArgs a) {
       void* out = reinterpret_cast<void*>(readlane_u64(ti.dst, 8 * g));
       const uint64_t ucnt = readlane_u64(ti.cnt, 8 * g);
       switch (__builtin_amdgcn_readlane(ti.fmt, 8 * g)) {
-        case 0: batch_tile<0, 2>(out, ucnt, u, k, a.lane); break;
-        case 1: batch_tile<1, 2>(out, ucnt, u, k, a.lane); break;
-        default: batch_tile<2, 2>(out, ucnt, u, k, a.lane); break;
+        case 0: batch_tile<0, 2, 16>(out, ucnt, u, k, a.lane); break;
+        case 1: batch_tile<1, 2, 16>(out, ucnt, u, k, a.lane); break;
+        default: batch_tile<2, 2, 16>(out, ucnt, u, k, a.lane); break;
       }
     }
   }
@@ -630,11 +634,10 @@ __global__ __launch_bounds__(256) void k_advance(AdvanceArgs a) {
 
 template <int FMT, int PL>
 hipError_t fill_one(const FillArgs& a, int block_threads, hipStream_t stream) {
-  const uint64_t waves = (a.n + a.tile - 1) / a.tile;
+  if (a.n == 0) return hipSuccess;
+  const uint64_t waves = fill_waves(a.n, a.head, a.tile);
   const uint64_t wpb = (uint64_t)block_threads / 64;
-  uint64_t blocks = (waves + wpb - 1) / wpb;
-  if (blocks == 0 && a.peel) blocks = 1;  // only the peeled outputs
-  if (blocks == 0) return hipSuccess;
+  const uint64_t blocks = (waves + wpb - 1) / wpb;
   if (blocks * (uint64_t)block_threads > 0xffffffffull) return hipErrorInvalidConfiguration;  // 32-bit grid
   hipLaunchKernelGGL((k_fill<FMT, PL>), dim3((uint32_t)blocks), dim3(block_threads), 0, stream, a);
   return hipGetLastError();

@@ -43,19 +43,24 @@ struct StreamStart {
   uint64_t offset;    // jump applied before output 0
 };
 
-// One fill launch over a single stream: peel_out[i] (i < peel) is stream
-// position start.offset + i and out[i] is position start.offset + peel + i.
-// Each wave produces `tile` consecutive outputs of `out`.
+// One fill launch over a single stream: out[i] is stream position
+// start.offset + i.  Wave 0 produces outputs [0, head + tile) and wave w >= 1
+// outputs [head + w tile, head + (w + 1) tile), so every wave after the first
+// starts on a 1 KiB boundary (head: the outputs before the first one).
 struct FillArgs {
-  void* out;                // 16-byte aligned unless outputs-per-lane is 1
+  void* out;                // element-aligned
   uint64_t n;
   uint64_t tile;            // multiple of 64 * outputs-per-lane
-  uint32_t peel;            // outputs written to peel_out[0..peel) before out[0] (at most 3)
-  void* peel_out;
-  StreamStart start;        // the state, and the position (offset) of the first output
+  uint64_t head;            // outputs before the first 1 KiB boundary of out
+  StreamStart start;        // the state, and the position (offset) of out[0]
   const CoefRecord* lane;
   const CoefRecord* jump;
 };
+
+// Waves of a fill of n >= 1 outputs.
+__host__ __device__ inline uint64_t fill_waves(uint64_t n, uint64_t head, uint64_t tile) {
+  return n > head ? (n - head + tile - 1) / tile : 1;
+}
 
 // A segment of the batched kernel: `count` outputs whose first one is
 // `offset` positions after the state of the segment's stream, written to

@@ -41,7 +41,6 @@ const JumpTable& host_jump_table();
 int device_context(int device, DeviceContext** out);
 bool state_valid(const uint32_t st[8]);
 int format_size(int format);
-int pick_per_lane(int format, const void* p);
 
 // Fills dptr[0..n) with stream positions offset..offset+n-1 of the stream
 // whose state is `state`; stream-ordered.  block_threads/tile/per_lane 0 =
