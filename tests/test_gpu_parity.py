@@ -59,7 +59,7 @@ def test_ragged_sizes(gpu, oracle_mod, n, fmt):
 @pytest.mark.parametrize("pad", [1, 2, 3])
 @pytest.mark.parametrize("fmt", [0, 2])
 def test_unaligned_outputs(gpu, oracle_mod, pad, fmt):
-    """Outputs that are only element-aligned take the narrow-store path."""
+    """Outputs that are only element-aligned: the waves peel to 1 KiB."""
     n = 100003
     say("unaligned", pad, fmt)
     g = gpu.mrg8_cuda(SEED)
@@ -79,6 +79,35 @@ def test_tiny_unaligned_outputs(gpu, oracle_mod, n, pad):
         want, st = oracle_mod.generate(oracle_mod.seed_state(SEED), n, fmt)
         assert np.array_equal(got.view(np.uint8), want.view(np.uint8))
         assert np.array_equal(g.get_state(), st)
+
+
+@pytest.mark.parametrize("fmt", [0, 2])
+@pytest.mark.parametrize("off_elems", [1, 2, 63, 64, 65, 127, 128, 129, 255])
+@pytest.mark.parametrize("n", [50, 127, 200, 3000, 70001])
+def test_peel_to_1kib_boundary(gpu, oracle_mod, fmt, off_elems, n):
+    """Destinations at element offsets within a 1 KiB line: wave 0 writes the
+    outputs before the first 1 KiB boundary one per lane (up to 63 per round,
+    so several rounds), the waves after it start on 1 KiB boundaries; the
+    bytes around the destination stay untouched."""
+    from mrg8_amd.hip import DeviceBuffer
+
+    esz = 8 if fmt != 2 else 4
+    if fmt == 0 and off_elems >= 128:
+        pytest.skip("offsets of 128 doubles and more are whole 1 KiB lines")
+    say("peel-1kib", fmt, off_elems, n)
+    buf = DeviceBuffer(n * esz + 4096)
+    buf.fill_byte(0xCD)
+    base = (buf.ptr + 1023) // 1024 * 1024
+    g = gpu.mrg8_cuda(SEED + off_elems)
+    g.generate(base + off_elems * esz, n, fmt, block_threads=256, tile=512)
+    g.synchronize()
+    host = buf.copy_to_host(np.uint8, n * esz + 4096)
+    start = base - buf.ptr + off_elems * esz
+    want, st = oracle_mod.generate(oracle_mod.seed_state(SEED + off_elems), n, fmt)
+    assert host[start:start + n * esz].tobytes() == want.tobytes()
+    assert (host[:start] == 0xCD).all() and (host[start + n * esz:] == 0xCD).all()
+    assert np.array_equal(g.get_state(), st)
+    buf.free()
 
 
 @pytest.mark.parametrize("fmt,per_lane", [(0, 1), (0, 2), (0, 4), (1, 4), (2, 1), (2, 2), (2, 4)])
