@@ -11,7 +11,8 @@
 //   * The next state is the last eight outputs, read back with v_readlane;
 //     no LDS and no barrier in the loop.
 //   * Outputs leave through coalesced non-temporal vector stores (16 bytes
-//     per lane once the destination is 16-byte aligned).
+//     per lane, 1 KiB per wave store); a run first writes the outputs before
+//     its first 1 KiB boundary one per lane.
 //   * A wave reaches its first position with a jump: each non-zero byte d
 //     at position k of the 64-bit offset is one product with A^(d 256^k)
 //     (at most 8 steps).  Eight lane groups of eight lanes jump eight states
@@ -324,11 +325,12 @@ __device__ __forceinline__ void peel_outputs(typename Out<FMT>::T* __restrict__ 
 // Byte alignment of the wide-store loop's wave stores: 64 lanes x 16 bytes.
 constexpr uintptr_t kStoreAlign = 1024;
 
-// Writes the outputs of a run before its first kStoreAlign boundary (at most
-// 63 per round, one per lane) and advances dst, cnt and the state past them,
-// so that every 1 KiB wave store of the loop that follows covers whole cache
-// lines: on MI355X a 2^28-double fill whose destination was 8 or 16 bytes off
-// such a boundary took 471-473 us against 342-347 us.
+// Writes the outputs of a run before its first ALIGN-byte boundary (at most
+// 63 per round, one per lane) and advances dst, cnt and the state past them.
+// With ALIGN = kStoreAlign every 1 KiB wave store of the loop that follows
+// covers whole cache lines: on MI355X a 2^28-double fill whose destination
+// was 8 or 16 bytes off such a boundary took 471-473 us with a 16-byte peel,
+// 346-350 us with this one (aligned: 342 us).
 template <int FMT, uintptr_t ALIGN>
 __device__ __forceinline__ void peel_to_boundary(typename Out<FMT>::T*& dst, uint64_t& cnt, uint32_t s[8],
                                                  const CoefRecord* __restrict__ lane_rec) {
@@ -429,7 +431,9 @@ __global__ __launch_bounds__(1024) void k_fill(FillArgs a) {
 }
 
 // One tile of a batch segment at dst (generate_run peels the outputs before
-// the first 1 KiB boundary; only tile 0 of a segment can have any).
+// the first ALIGN-byte boundary; only tile 0 of a segment can have any).  The
+// persistent kernel passes ALIGN = 16: with the 1 KiB peel it ran 10 % slower
+// on the 4096-stream batch.
 template <int FMT, int PL, uintptr_t ALIGN = kStoreAlign>
 __device__ __forceinline__ void batch_tile(void* out, uint64_t cnt, uint32_t s[8], const Coef (&k)[PL],
                                            const CoefRecord* __restrict__ lane_rec) {
