@@ -183,11 +183,19 @@ int format_size(int format) {
 // iterations) so that every SIMD still gets waves.
 void default_shape(const DeviceContext& ctx, uint64_t n, int format, int per_lane, int* block, uint64_t* tile) {
   const uint64_t quantum = 64ull * per_lane;
-  *block = format == MRG8_U32 ? 256 : 1024;
   uint64_t t = format == MRG8_U32 ? 8192 : 2048;
-  const uint64_t min_waves = (uint64_t)ctx.num_cu * 4 * 4;  // four waves per SIMD
+  const uint64_t cu = (uint64_t)ctx.num_cu;
+  const uint64_t min_waves = cu * 4 * 4;  // four waves per SIMD
   while (t > 4 * quantum && (n + t - 1) / t < min_waves) t /= 2;
   *tile = std::max<uint64_t>(quantum, t / quantum * quantum);
+  // Double fills take 1024-thread blocks once every CU gets 128 waves or
+  // more; smaller fills spread over more, smaller blocks (on MI355X, against
+  // 1024-thread blocks: 2^20 and 2^21 doubles 15-44 % faster, 2^22..2^25
+  // 0-4 % faster, 2^26 2-3 % slower in 512-thread blocks).
+  int b = format == MRG8_U32 ? 256 : 1024;
+  const uint64_t waves = (n + *tile - 1) / *tile;
+  if (format != MRG8_U32 && waves < cu * 128) b = waves <= cu * 4 ? 256 : 512;
+  *block = b;
 }
 
 int fill_device(DeviceContext& ctx, const uint32_t state[8], uint64_t offset, void* dptr, uint64_t n, int format,
