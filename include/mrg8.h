@@ -114,8 +114,8 @@ int mrg8_advance_state_bits(const uint32_t state[8], const short bits[200], uint
  * (n is 64-bit; the reference takes int and jumps only 32 bits on device). */
 int mrg8_generate(mrg8_gen_t g, void* dptr, uint64_t n, int format, void* stream);
 /* Same, with an explicit launch shape: threads per block (64..1024, multiple
- * of 64), outputs per wave (rounded up to 64 * outputs per lane) and outputs
- * per lane (1, 2 or 4); 0 means automatic for each.  The output is
+ * of 64), outputs per wave (at most n, rounded up to 64 * outputs per lane)
+ * and outputs per lane (1, 2 or 4); 0 means automatic for each.  The output is
  * identical for every shape. */
 int mrg8_generate_shape(mrg8_gen_t g, void* dptr, uint64_t n, int format, void* stream,
                         int block_threads, uint64_t outputs_per_wave, int outputs_per_lane);
@@ -220,7 +220,7 @@ typedef struct mrg8_batch_config {
   int32_t blocks_per_cu;  /* 0: one tile per wave (grid from the tile count; best for long
                              segments); > 0: persistent grid of CUs * blocks_per_cu blocks
                              (best for many short segments with long jumps) */
-  uint64_t tile_outputs;  /* outputs per work item, multiple of 256 */
+  uint64_t tile_outputs;  /* outputs per work item, multiple of 256, at most 2^40 */
 } mrg8_batch_config;
 
 /* A batch object owns device descriptor and tile tables sized for
@@ -256,7 +256,8 @@ typedef struct mrg8_server_config {
   uint32_t max_inflight;       /* requests admitted at once */
   uint64_t max_output_bytes;   /* admission cap on output-buffer bytes in flight */
   uint64_t step_outputs;       /* target outputs per launch (0 = automatic) */
-  uint64_t tile_outputs;       /* outputs per work item (0 = automatic) */
+  uint64_t tile_outputs;       /* outputs per work item, multiple of 256, at most 2^40
+                                  (0 = automatic) */
   int32_t block_threads;       /* 0 = automatic */
   int32_t blocks_per_cu;       /* 0 = automatic */
   uint32_t steps_in_flight;    /* launches in flight at once, 1..8 (0 = 3) */

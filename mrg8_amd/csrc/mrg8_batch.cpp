@@ -43,8 +43,8 @@ int validate_config(const mrg8_batch_config& c) {
   if (c.blocks_per_cu < 0 || c.blocks_per_cu * (c.block_threads / 64) > 32)
     return fail(MRG8_ERR_ARG, "blocks_per_cu %d must be 0 (one tile per wave) or give at most 32 waves per CU",
                 c.blocks_per_cu);
-  if (c.tile_outputs < 256 || c.tile_outputs % 256)
-    return fail(MRG8_ERR_ARG, "tile_outputs must be a positive multiple of 256");
+  if (c.tile_outputs < 256 || c.tile_outputs % 256 || c.tile_outputs > kMaxTile)
+    return fail(MRG8_ERR_ARG, "tile_outputs must be a positive multiple of 256, at most 2^40");
   return MRG8_OK;
 }
 

@@ -229,6 +229,7 @@ int fill_device(DeviceContext& ctx, const uint32_t state[8], uint64_t offset, vo
     return fail(MRG8_ERR_ARG, "block_threads %d must be a multiple of 64 in [64, 1024]", block_threads);
   const uint64_t quantum = 64ull * pl;
   if (tile == 0) tile = def_tile;
+  if (tile > n) tile = n;  // one wave takes everything (and the rounding below cannot wrap)
   if (tile % quantum) tile = (tile + quantum - 1) / quantum * quantum;
   // Wave 0 also takes the outputs before the first 1 KiB boundary, so every
   // other wave starts on one and writes whole cache lines (a 2^28-double fill

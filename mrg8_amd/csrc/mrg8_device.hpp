@@ -23,6 +23,10 @@ constexpr int kIdentityRec = kJumpPositions * kJumpValues * 8;
 constexpr int kAdvanceWords = 4;
 constexpr int kWidePositions = 8 * kAdvanceWords;
 
+// Largest batch tile (outputs): keeps head + (k + 1) * tile of the tile
+// bounds within 64 bits.
+constexpr uint64_t kMaxTile = 1ull << 40;
+
 // Per-device constant tables.
 struct DeviceTables {
   CoefRecord* lane = nullptr;  // kLaneRows records: row 0 of A^(k+1), support 0xff

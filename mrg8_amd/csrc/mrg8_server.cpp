@@ -290,7 +290,8 @@ extern "C" int mrg8_server_create(int device, const mrg8_server_config* cfg, uin
   if (!s->cfg.steps_in_flight) s->cfg.steps_in_flight = 3;
   if (!s->cfg.age_limit_us) s->cfg.age_limit_us = 1000;
   if (s->cfg.steps_in_flight > kMaxSteps) return fail(MRG8_ERR_ARG, "steps_in_flight must be at most %u", kMaxSteps);
-  if (s->cfg.tile_outputs % 256 || s->cfg.block_threads % 64 || s->cfg.block_threads > 1024 ||
+  if (s->cfg.tile_outputs % 256 || s->cfg.tile_outputs > kMaxTile || s->cfg.block_threads % 64 ||
+      s->cfg.block_threads > 1024 ||
       s->cfg.blocks_per_cu * (s->cfg.block_threads / 64) > 32)
     return fail(MRG8_ERR_ARG, "invalid launch configuration");
   s->launch.block_threads = s->cfg.block_threads;

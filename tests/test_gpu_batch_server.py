@@ -216,6 +216,10 @@ def test_batch_errors(gpu):
         b.set([([1] * 8, 0, 10**9, 8, 0)])  # too many tiles
     with pytest.raises(Mrg8Error):
         b.set_config(100, 1, 1024)
+    with pytest.raises(Mrg8Error):
+        b.set_config(256, 0, 1 << 41)  # tile above 2^40
+    with pytest.raises(Mrg8Error):
+        b.set_config(256, 0, 2**64 - 256)
 
 
 @pytest.mark.parametrize("shape", [None, (1024, 0, 4096)], ids=["default", "tile-per-wave"])

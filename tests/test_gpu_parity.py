@@ -133,7 +133,9 @@ def test_two_store_layout_ragged(gpu, oracle_mod, n, fmt, pad):
     assert np.array_equal(g.get_state(), st)
 
 
-@pytest.mark.parametrize("block,tile", [(64, 128), (256, 1024), (512, 4096), (1024, 65536), (256, 1 << 20)])
+# tiles of n and above (2^64 - 1 included) give one wave the whole fill
+@pytest.mark.parametrize("block,tile", [(64, 128), (256, 1024), (512, 4096), (1024, 65536), (256, 1 << 20),
+                                        (512, 1_000_003), (256, 2**64 - 1)])
 def test_launch_shapes_identical(gpu, oracle_mod, block, tile):
     n = 1_000_003
     say("shape", block, tile)
