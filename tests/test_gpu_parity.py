@@ -110,6 +110,43 @@ def test_peel_to_1kib_boundary(gpu, oracle_mod, fmt, off_elems, n):
     buf.free()
 
 
+def test_random_fills_default_shape(gpu, oracle_mod):
+    """60 seeded random cases with the library's own launch shape: sizes from
+    1 to 2^22 (log-uniform, so every block-size and tile rule of the default
+    shape is taken), every format, destinations at random element offsets
+    within a 4 KiB line, random states (a few with zero words); outputs, the
+    bytes around them and the final state against the oracle."""
+    from mrg8_amd.hip import DeviceBuffer
+
+    rng = np.random.default_rng(2024)
+    p = 2**31 - 1
+    buf = DeviceBuffer((1 << 22) * 8 + 16384)
+    try:
+        base = (buf.ptr + 4095) // 4096 * 4096
+        for case in range(60):
+            n = int(np.exp(rng.uniform(0, np.log(1 << 22))))
+            fmt = int(rng.integers(0, 3))
+            esz = 4 if fmt == 2 else 8
+            off = int(rng.integers(0, 4096 // esz)) * esz
+            st = rng.integers(1, p + 1, size=8, dtype=np.int64).astype(np.uint32)
+            if case % 10 == 3:
+                st[rng.integers(0, 8, size=3)] = 0
+            say("random fill", case, n, fmt, off, st.tolist())
+            buf.fill_byte(0xCD)
+            g = gpu.mrg8_cuda(1)
+            g.set_state(st)
+            g.generate(base + off, n, fmt)
+            g.synchronize()
+            lo = base - buf.ptr + off
+            host = buf.copy_to_host(np.uint8, lo + n * esz + 64)
+            want, st_after = oracle_mod.generate(st, n, fmt)
+            assert host[lo:lo + n * esz].tobytes() == want.tobytes(), case
+            assert (host[:lo] == 0xCD).all() and (host[lo + n * esz:] == 0xCD).all(), case
+            assert np.array_equal(g.get_state(), st_after), case
+    finally:
+        buf.free()
+
+
 @pytest.mark.parametrize("fmt,per_lane", [(0, 1), (0, 2), (0, 4), (1, 4), (2, 1), (2, 2), (2, 4)])
 def test_outputs_per_lane(gpu, oracle_mod, fmt, per_lane):
     n = 777_777
