@@ -180,6 +180,7 @@ __device__ __forceinline__ const CoefRecord* digit_rec(const CoefRecord* __restr
 // One step of the pipelined jumps below: lane r of group g computes row r of
 // the group's matrix (record c) times the group's state, and the group's
 // eight new words are gathered back.
+template <bool ZERO_WORDS = true>
 __device__ __forceinline__ void jump_step(uint32_t s[8], const Coef& c, int grp) {
   int32_t e[8];
   centre(s, e);
@@ -189,7 +190,7 @@ __device__ __forceinline__ void jump_step(uint32_t s[8], const Coef& c, int grp)
   uint32_t smin = s[0];
 #pragma unroll
   for (int j = 1; j < 8; ++j) smin = min(smin, s[j]);
-  if (smin == 0u) {
+  if (ZERO_WORDS && smin == 0u) {
     uint32_t nz = 0;
 #pragma unroll
     for (int j = 0; j < 8; ++j) nz |= (s[j] != 0u ? 1u : 0u) << j;
@@ -588,14 +589,6 @@ __global__ __launch_bounds__(1024) void k_batch_tiles(BatchArgs a) {
 
 // ---- bulk advance: many states, offsets of up to kAdvanceWords words --------
 
-// Digit at byte position k (wave-uniform) of a multiword offset.
-__device__ __forceinline__ uint32_t wide_digit(const uint64_t (&off)[kAdvanceWords], int k) {
-  uint64_t w = off[0];
-#pragma unroll
-  for (int i = 1; i < kAdvanceWords; ++i) w = (k >> 3) == i ? off[i] : w;
-  return (uint32_t)(w >> (8 * (k & 7))) & (uint32_t)kJumpValues;
-}
-
 __device__ __forceinline__ const CoefRecord* wide_rec(const AdvanceArgs& a, int k, uint32_t d, int row) {
   if (d == 0u) return a.jump + (kIdentityRec + row);
   return k < kJumpPositions ? a.jump + ((k * kJumpValues + (int)d - 1) * 8 + row)
@@ -603,33 +596,80 @@ __device__ __forceinline__ const CoefRecord* wide_rec(const AdvanceArgs& a, int 
 }
 
 // Eight lanes per state (lane r of a group computes row r, as in the jumps
-// above); byte positions of the offset in order, positions where no state of
-// the wave has a digit skipped, the next position's records fetched while
-// the current step computes.  A state is read before it is written, so `in`
-// and `out` may be the same buffer.
+// above); byte positions of the offset in order, each digit extracted once,
+// positions where no state of the wave has a digit skipped, the next
+// position's records fetched while the current step computes.  Waves whose
+// states have no zero word skip the zero-product test and keep the words
+// centred between steps (2^20 states, 200-bit offsets: 400 us against 446 us
+// for the plain loop).  A state is read before it is written, so `in` and
+// `out` may be the same buffer.
+// jump_step for states with no zero word, on centred words e = s - 2^30:
+// each lane centres its one new word before the gather instead of every lane
+// centring eight after it.
+__device__ __forceinline__ void jump_step_centred(int32_t e[8], const Coef& c, int grp) {
+  const int32_t ey = (int32_t)(dot_residue(c, e) + 1u - kCenter);
+#pragma unroll
+  for (int j = 0; j < 8; ++j) e[j] = __builtin_amdgcn_ds_bpermute((grp * 8 + j) * 4, ey);
+}
+
+template <bool ZERO_WORDS>
+__device__ __forceinline__ void advance_digits(uint32_t s[8], const uint64_t (&off)[kAdvanceWords],
+                                               const AdvanceArgs& a, int grp, int row) {
+  const int npos = 8 * (int)a.words;
+  int32_t e[8];
+  if (!ZERO_WORDS) centre(s, e);
+  uint32_t d = (uint32_t)off[0] & (uint32_t)kJumpValues;
+  Coef c = load_coef11(wide_rec(a, 0, d, row));
+#pragma unroll
+  for (int w = 0; w < kAdvanceWords; ++w) {
+    if (w >= (int)a.words) break;
+    for (int b = 0; b < 8; ++b) {
+      const int k = 8 * w + b;
+      uint32_t dn;
+      if (b < 7)
+        dn = (uint32_t)(off[w] >> (8 * (b + 1))) & (uint32_t)kJumpValues;
+      else
+        dn = w + 1 < kAdvanceWords ? (uint32_t)off[w + 1 < kAdvanceWords ? w + 1 : w] & (uint32_t)kJumpValues : 0u;
+      const bool last = k + 1 == npos;
+      if (last) dn = 0u;
+      const Coef next = load_coef11(wide_rec(a, last ? 0 : k + 1, dn, row));
+      if (__ballot(d != 0u)) {
+        if (ZERO_WORDS)
+          jump_step<true>(s, c, grp);
+        else
+          jump_step_centred(e, c, grp);
+      }
+      c = next;
+      d = dn;
+    }
+  }
+  if (!ZERO_WORDS) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) s[j] = (uint32_t)e[j] + kCenter;
+  }
+}
+
 __global__ __launch_bounds__(256) void k_advance(AdvanceArgs a) {
   const int lane = threadIdx.x & 63;
   const int grp = lane >> 3, row = lane & 7;
   const uint64_t item = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 3;
   const bool live = item < a.n;
   uint32_t s[8];
-  uint64_t off[kAdvanceWords] = {};
-  if (live) {
+  uint64_t off[kAdvanceWords];
 #pragma unroll
-    for (int j = 0; j < 8; ++j) s[j] = a.in[8 * item + j];
-    for (uint32_t w = 0; w < a.words; ++w) off[w] = a.offsets[item * a.words + w];
-  } else {
+  for (int w = 0; w < kAdvanceWords; ++w) off[w] = live && w < (int)a.words ? a.offsets[item * a.words + w] : 0;
+  uint32_t smin = 1u;
 #pragma unroll
-    for (int j = 0; j < 8; ++j) s[j] = 1u;
+  for (int j = 0; j < 8; ++j) {
+    s[j] = live ? a.in[8 * item + j] : 1u;
+    smin = min(smin, s[j]);
   }
-  const int npos = 8 * (int)a.words;
-  Coef c = load_coef11(wide_rec(a, 0, wide_digit(off, 0), row));
-  for (int k = 0; k < npos; ++k) {
-    const bool last = k + 1 == npos;
-    const Coef next = load_coef11(wide_rec(a, last ? 0 : k + 1, last ? 0u : wide_digit(off, k + 1), row));
-    if (__ballot(wide_digit(off, k) != 0u)) jump_step(s, c, grp);
-    c = next;
-  }
+  // a state with no zero word never gets one (no row of a power of A is
+  // zero), so the zero-product test is needed only by waves that start with one
+  if (__ballot(smin == 0u))
+    advance_digits<true>(s, off, a, grp, row);
+  else
+    advance_digits<false>(s, off, a, grp, row);
   if (live && row == 0) {
 #pragma unroll
     for (int j = 0; j < 8; ++j) a.out[8 * item + j] = s[j];
