@@ -13,6 +13,8 @@ last `--reps` dispatches of each kind):
   4. batched launch: 4096 streams, log-uniform lengths, float64, one
      2048-output tile per wave, 1024-thread blocks                (--batched)
   5. jump-heavy batched launch: 65536 x (63-bit jump + 256 uint32) (--batched)
+  6. bulk advance of 2^20 device-resident states by random 200-bit offsets,
+     then by random 64-bit offsets                                 (--advance)
 """
 from __future__ import annotations
 
@@ -30,6 +32,7 @@ def main():
     ap.add_argument("--reps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--batched", action="store_true")
+    ap.add_argument("--advance", action="store_true")
     args = ap.parse_args()
 
     import mrg8_amd
@@ -76,6 +79,24 @@ def main():
             j.run()
         hip.synchronize()
         out.free()
+    if args.advance:
+        rng = np.random.default_rng(7)
+        ns = 1 << 20
+        states = rng.integers(1, 2**31 - 1, size=(ns, 8), dtype=np.int64).astype(np.uint32)
+        d_in, d_out = hip.DeviceBuffer(ns * 32), hip.DeviceBuffer(ns * 32)
+        d_in.copy_from_host(states)
+        for words in (4, 1):
+            offs = rng.integers(0, 2**63, size=(ns, words), dtype=np.uint64) * np.uint64(2) + np.uint64(1)
+            if words == 4:
+                offs[:, 3] &= np.uint64(0xFF)  # 200-bit offsets
+            d_off = hip.DeviceBuffer(offs.nbytes)
+            d_off.copy_from_host(offs)
+            for _ in range(args.warmup + args.reps):
+                mrg8_amd.advance_states(d_in, d_off, words, d_out, ns)
+            hip.synchronize()
+            d_off.free()
+        d_in.free()
+        d_out.free()
     print("profile workload done", flush=True)
 
 
