@@ -138,10 +138,15 @@ int mrg8_tune_fill(int device, int format, uint64_t n, const char* cache_path, m
 
 /* mrg8_cuda::mrg8_outer_32(d_ran, n[, TNUM[, BS]]) (cuda/mrg8_cuda.cu:414-436):
  * blocking, [0, 1) doubles (MRG8_F64_01).  BS/TNUM keep the reference's
- * meaning (threads per block / total threads) and pick the launch shape. */
+ * meaning (threads per block / total threads) and pick the launch shape;
+ * TNUM = 0 takes the library's own shape (what the C++ shim's two-argument
+ * form passes: the reference's default of 64 * 1024 threads gives every wave
+ * n / 1024 outputs), BS = 0 the reference's 64.  The outputs are the same
+ * for every shape. */
 int mrg8_outer_32(mrg8_gen_t g, double* d_ran, int n, int BS, int TNUM);
 /* mrg8_cuda::mrg8_inner(d_ran, n, TNUM) (cuda/mrg8_cuda.cu:402-412): blocking,
- * [0, 1) doubles; the same sequence as mrg8_outer_32. */
+ * [0, 1) doubles; the same sequence as mrg8_outer_32 (TNUM = 0: the
+ * library's own shape). */
 int mrg8_inner(mrg8_gen_t g, double* d_ran, int n, int TNUM);
 /* mrg8_cuda::state_cpy_DtH (cuda/mrg8_cuda.cu:72-79).  The host copy of the
  * state is always current here; this waits for the generator's outstanding

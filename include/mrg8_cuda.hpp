@@ -96,7 +96,11 @@ class mrg8_cuda : public mrg8 {
   void mrg8_inner(double* d_ran, const int n, const int TNUM) {
     mrg8_amd_detail::check(::mrg8_inner(g_, d_ran, n, TNUM), "mrg8_inner");
   }
-  void mrg8_outer_32(double* d_ran, const int n) { mrg8_outer_32(d_ran, n, 64 * 1024); }
+  // the reference's default of 64 * 1024 threads (cuda/mrg8_cuda.cu:433-436)
+  // was a CUDA launch choice; here the library picks the shape (same outputs)
+  void mrg8_outer_32(double* d_ran, const int n) {
+    mrg8_amd_detail::check(::mrg8_outer_32(g_, d_ran, n, 0, 0), "mrg8_outer_32");
+  }
   void mrg8_outer_32(double* d_ran, const int n, const int TNUM) { mrg8_outer_32(d_ran, n, 64, TNUM); }
   void mrg8_outer_32(double* d_ran, const int n, const int BS, const int TNUM) {
     mrg8_amd_detail::check(::mrg8_outer_32(g_, d_ran, n, BS, TNUM), "mrg8_outer_32");

@@ -115,8 +115,10 @@ class mrg8_cuda:
             capi.call("mrg8_generate", self._h, ctypes.c_void_p(_ptr(dptr)), ctypes.c_uint64(n), fmt,
                       _stream(stream))
 
-    def mrg8_outer_32(self, d_ran, n: int, BS: int = 64, TNUM: int = 64 * 1024) -> None:
-        """cuda/mrg8_cuda.cu:414-436: blocking, [0,1) doubles."""
+    def mrg8_outer_32(self, d_ran, n: int, BS: int = 0, TNUM: int = 0) -> None:
+        """cuda/mrg8_cuda.cu:414-436: blocking, [0,1) doubles.  TNUM = 0 (the
+        default) takes the library's own launch shape; BS = 0 means 64, the
+        reference's threads per block when only TNUM is given."""
         capi.call("mrg8_outer_32", self._h, ctypes.c_void_p(_ptr(d_ran)), n, BS, TNUM)
 
     def mrg8_inner(self, d_ran, n: int, TNUM: int = 64 * 1024) -> None:

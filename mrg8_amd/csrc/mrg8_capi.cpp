@@ -486,12 +486,19 @@ extern "C" int mrg8_synchronize(mrg8_gen_t g) {
 }
 
 // The reference's launchers take int n, BS (threads per block) and TNUM
-// (total threads); they block until the kernel is done.
+// (total threads); they block until the kernel is done.  TNUM = 0 takes the
+// library's own launch shape, BS = 0 the reference's 64 threads per block.
 static int drop_in_fill(mrg8_gen_t g, double* d, int n, int bs, int tnum) {
   if (!g) return fail(MRG8_ERR_ARG, "null generator");
   if (n < 0) return fail(MRG8_ERR_ARG, "n must be non-negative");
-  if (bs <= 0 || tnum <= 0) return fail(MRG8_ERR_ARG, "BS and TNUM must be positive");
+  if (bs < 0 || tnum < 0) return fail(MRG8_ERR_ARG, "BS and TNUM must be non-negative");
   if (n == 0) return MRG8_OK;
+  if (tnum == 0) {
+    int rc = generate_impl(g, d, (uint64_t)n, MRG8_F64_01, nullptr, 0, 0, 0);
+    if (rc) return rc;
+    return mrg8_synchronize(g);
+  }
+  if (bs == 0) bs = 64;
   // TNUM threads in waves of 64: each wave covers n / (TNUM / 64) outputs.
   const uint64_t waves = std::max<uint64_t>(1, (uint64_t)tnum / 64);
   const uint64_t quantum = 64ull * 4;  // 4 outputs per lane

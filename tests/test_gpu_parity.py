@@ -270,12 +270,16 @@ def test_drop_in_outer_inner(gpu, oracle_mod):
     n = 1 << 20
     say("drop-in")
     buf = DeviceBuffer(n * 8)
-    for call in ("outer", "inner"):
+    calls = {
+        "outer": lambda g: g.mrg8_outer_32(buf, n, 64, 65536),
+        "outer, TNUM only (BS 64)": lambda g: g.mrg8_outer_32(buf, n, TNUM=65536),
+        "outer, library shape": lambda g: g.mrg8_outer_32(buf, n),
+        "inner": lambda g: g.mrg8_inner(buf, n, 65536),
+        "inner, library shape": lambda g: g.mrg8_inner(buf, n, 0),
+    }
+    for call, fn in calls.items():
         g = gpu.mrg8_cuda(SEED)
-        if call == "outer":
-            g.mrg8_outer_32(buf, n, 64, 65536)
-        else:
-            g.mrg8_inner(buf, n, 65536)
+        fn(g)
         got = buf.copy_to_host(np.float64, n)
         want, st = oracle_mod.generate(oracle_mod.seed_state(SEED), n, 1)
         assert np.array_equal(got.view(np.uint64), want.view(np.uint64)), call
