@@ -398,7 +398,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the generator needs no "
-              f"cross-GPU exchange; see DESIGN.md for how it shards)", file=sys.stderr)
+              f"cross-GPU exchange: each GPU would fill its own range of the sequence, its start state "
+              f"jumped ahead on the host)", file=sys.stderr)
         sys.exit(2)
 
     import mrg8_amd
