@@ -8,13 +8,19 @@
 //   * errors throw mrg8_amd_error instead of exit() / being ignored;
 //   * set_state() is the state the next device generation starts from
 //     (the reference's mrg8_cuda only updates its host copy);
-//   * generation sizes may exceed int through the uint64_t overloads.
+//   * generation sizes may exceed int through the uint64_t overloads, which
+//     take part only for arguments of exactly uint64_t, so a call that the
+//     reference's single int signature accepts (unsigned, long, size_t
+//     arguments) resolves to the int overload here too.
 #ifndef MRG8_AMD_CUDA_HPP
 #define MRG8_AMD_CUDA_HPP
 
 #include <cstdint>
+#include <iomanip>
+#include <iostream>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 
 #include "mrg8.h"
 
@@ -47,16 +53,32 @@ class mrg8 {
   void seed_init(const uint32_t seed_val) { mrg8_amd_detail::check(mrg8_seed_init(g_, seed_val), "seed_init"); }
   void get_state(uint32_t st[8]) const { mrg8_amd_detail::check(mrg8_get_state(g_, st), "get_state"); }
   void set_state(const uint32_t st[8]) { mrg8_amd_detail::check(mrg8_set_state(g_, st), "set_state"); }
+  // original/mrg8.cpp:41-54 (cuda/mrg8.cpp:50-68): the same text on std::cout
+  void print_state() const {
+    uint32_t st[8];
+    get_state(st);
+    std::cout << "(S[n-1], S[n-2], ..., S[n-8])= (";
+    for (int i = 0; i < 7; ++i) std::cout << std::setw(10) << st[i] << ", ";
+    std::cout << st[7] << ")" << std::endl;
+  }
+  void print_matrix(const uint32_t jm[8][8]) const {
+    for (int i = 0; i < 8; ++i) {
+      for (int j = 0; j < 8; ++j) std::cout << std::setw(10) << jm[i][j] << " ";
+      std::cout << std::endl;
+    }
+  }
 
-  void randint(uint32_t* iran, int n) { randint(iran, (uint64_t)(n < 0 ? 0 : n)); }
-  void randint(uint32_t* iran, uint64_t n) { mrg8_amd_detail::check(mrg8_randint(g_, iran, n), "randint"); }
+  void randint(uint32_t* iran, int n) { randint64(iran, (uint64_t)(n < 0 ? 0 : n)); }
+  template <typename N, typename std::enable_if<std::is_same<N, uint64_t>::value, int>::type = 0>
+  void randint(uint32_t* iran, N n) { randint64(iran, n); }
   uint32_t randint() {
     uint32_t v;
     mrg8_amd_detail::check(mrg8_randint1(g_, &v), "randint");
     return v;
   }
-  void rand(double* fran, int n) { rand(fran, (uint64_t)(n < 0 ? 0 : n)); }
-  void rand(double* fran, uint64_t n) { mrg8_amd_detail::check(mrg8_rand(g_, fran, n), "rand"); }
+  void rand(double* fran, int n) { rand64(fran, (uint64_t)(n < 0 ? 0 : n)); }
+  template <typename N, typename std::enable_if<std::is_same<N, uint64_t>::value, int>::type = 0>
+  void rand(double* fran, N n) { rand64(fran, n); }
   double rand() {
     double v;
     mrg8_amd_detail::check(mrg8_rand1(g_, &v), "rand");
@@ -84,6 +106,10 @@ class mrg8 {
 
  protected:
   mrg8_gen_t g_ = nullptr;
+
+ private:
+  void randint64(uint32_t* iran, uint64_t n) { mrg8_amd_detail::check(mrg8_randint(g_, iran, n), "randint"); }
+  void rand64(double* fran, uint64_t n) { mrg8_amd_detail::check(mrg8_rand(g_, fran, n), "rand"); }
 };
 
 // Device generation surface (cuda/mrg8_cuda.h:20-45).  Output is the [0, 1)
