@@ -48,6 +48,7 @@ extern "C" {
 #define MRG8_ERR_STATE (-6)     /* call not valid in the object's current state */
 #define MRG8_ERR_NOTFOUND (-7)  /* unknown stream or request id */
 #define MRG8_ERR_IO (-8)        /* file could not be read or written */
+#define MRG8_ERR_BLOCKED (-9)   /* no progress possible until the caller releases completed requests */
 
 /* Output formats. */
 #define MRG8_F64 0     /* double(x) * (1/p), (0, 1]: original/mrg8.cpp:231-239 (mrg8::rand) */
@@ -259,7 +260,8 @@ int mrg8_batch_autotune(mrg8_batch_t b, void* stream, const char* cache_path, mr
 typedef struct mrg8_server_config {
   uint32_t num_streams;        /* resident per-stream states in device memory */
   uint32_t max_inflight;       /* requests admitted at once */
-  uint64_t max_output_bytes;   /* admission cap on output-buffer bytes in flight */
+  uint64_t max_output_bytes;   /* admission cap on output-buffer bytes in flight (rounded down to a
+                                  multiple of 256, the arena's allocation granule; at least 256) */
   uint64_t step_outputs;       /* target outputs per launch (0 = automatic) */
   uint64_t tile_outputs;       /* outputs per work item, multiple of 256, at most 2^40
                                   (0 = automatic) */
@@ -294,11 +296,18 @@ int mrg8_server_get_stream(mrg8_server_t s, uint32_t stream_id, uint32_t state[8
 /* Queues a request for the next `count` outputs of stream `stream_id`; the
  * request is given the stream's next `count` positions at submit time, so
  * its outputs do not depend on what else is in flight.  Admission to the GPU
- * happens inside mrg8_server_progress as output memory and slots free up. */
+ * happens inside mrg8_server_progress as output memory and slots free up.
+ * A request whose output bytes, rounded up to the 256-byte granule, exceed
+ * the cap is MRG8_ERR_CAPACITY (it could never be admitted). */
 int mrg8_server_submit(mrg8_server_t s, uint32_t stream_id, uint64_t count, int format, uint64_t* request_id);
 /* Admits queued requests, launches work, and collects finished requests.
  * Up to `max` completions are written to `done`; *ndone gets the number.
- * `wait_ms` > 0 blocks up to that long for at least one completion. */
+ * `wait_ms` > 0 blocks up to that long for at least one completion.
+ * Admission is first come, first served: when nothing is on the device,
+ * nothing completed is left to report and the oldest queued request does not
+ * fit because completed requests the caller has not released hold the
+ * arena (or its slots), the call returns MRG8_ERR_BLOCKED at once instead of
+ * waiting: only mrg8_server_release can make progress possible. */
 int mrg8_server_progress(mrg8_server_t s, mrg8_completion* done, uint32_t max, uint32_t* ndone, int wait_ms);
 /* Returns a completed request's output buffer to the admission pool. */
 int mrg8_server_release(mrg8_server_t s, uint64_t request_id);

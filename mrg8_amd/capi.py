@@ -13,6 +13,7 @@ LIB_PATH = os.path.join(HERE, "libmrg8_amd.so")
 
 OK = 0
 ERR_ARG, ERR_HIP, ERR_NOMEM, ERR_NODEVICE, ERR_CAPACITY, ERR_STATE, ERR_NOTFOUND, ERR_IO = -1, -2, -3, -4, -5, -6, -7, -8
+ERR_BLOCKED = -9
 F64, F64_01, U32 = 0, 1, 2
 
 

@@ -310,6 +310,7 @@ extern "C" const char* mrg8_strerror(int status) {
     case MRG8_ERR_STATE: return "invalid state for this call";
     case MRG8_ERR_NOTFOUND: return "not found";
     case MRG8_ERR_IO: return "I/O error";
+    case MRG8_ERR_BLOCKED: return "blocked until completed requests are released";
     default: return "unknown status";
   }
 }

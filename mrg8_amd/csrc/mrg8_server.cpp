@@ -44,6 +44,8 @@ namespace {
 constexpr uint32_t kMaxSteps = 8;
 constexpr size_t kAlign = 256;
 
+size_t granule_bytes(size_t bytes) { return (bytes + kAlign - 1) / kAlign * kAlign; }
+
 double wall_now() {
   using namespace std::chrono;
   return duration<double>(steady_clock::now().time_since_epoch()).count();
@@ -59,7 +61,7 @@ class Arena {
     used_ = 0;
   }
   bool alloc(size_t bytes, size_t* off) {
-    bytes = (bytes + kAlign - 1) / kAlign * kAlign;
+    bytes = granule_bytes(bytes);
     for (auto it = free_.begin(); it != free_.end(); ++it) {
       if (it->second < bytes) continue;
       *off = it->first;
@@ -72,7 +74,7 @@ class Arena {
     return false;
   }
   void release(size_t off, size_t bytes) {
-    bytes = (bytes + kAlign - 1) / kAlign * kAlign;
+    bytes = granule_bytes(bytes);
     used_ -= bytes;
     auto it = free_.emplace(off, bytes).first;
     auto next = std::next(it);
@@ -270,13 +272,16 @@ extern "C" int mrg8_server_create(int device, const mrg8_server_config* cfg, uin
   if (!cfg || !out) return fail(MRG8_ERR_ARG, "null argument");
   *out = nullptr;
   if (cfg->num_streams == 0) return fail(MRG8_ERR_ARG, "num_streams must be positive");
-  if (cfg->max_output_bytes == 0) return fail(MRG8_ERR_ARG, "max_output_bytes must be positive");
+  if (cfg->max_output_bytes < kAlign) return fail(MRG8_ERR_ARG, "max_output_bytes must be at least %zu", kAlign);
   DeviceContext* ctx = nullptr;
   int rc = device_context(device, &ctx);
   if (rc) return rc;
   auto s = std::make_unique<mrg8_server>();
   s->ctx = ctx;
   s->cfg = *cfg;
+  // the arena hands out whole granules: a cap that is not a multiple would
+  // accept requests whose rounded size can never fit
+  s->cfg.max_output_bytes = s->cfg.max_output_bytes / kAlign * kAlign;
   if (!s->cfg.max_inflight) s->cfg.max_inflight = 4096;
   // Defaults measured on MI355X with the log-uniform request mix at 490-680
   // G samples/s offered: 4096-output tiles, one 256-thread block per CU (the
@@ -417,7 +422,8 @@ extern "C" int mrg8_server_submit(mrg8_server_t s, uint32_t stream_id, uint64_t 
   const int esz = format_size(format);
   if (!esz) return fail(MRG8_ERR_ARG, "unknown format %d", format);
   if (count == 0) return fail(MRG8_ERR_ARG, "count must be positive");
-  if (count > s->cfg.max_output_bytes / (uint64_t)esz)
+  if (count > s->cfg.max_output_bytes / (uint64_t)esz ||
+      granule_bytes(count * (uint64_t)esz) > s->cfg.max_output_bytes)
     return fail(MRG8_ERR_CAPACITY, "request of %llu outputs exceeds the output memory cap",
                 (unsigned long long)count);
   if (count > UINT64_MAX - s->cursor[stream_id]) return fail(MRG8_ERR_CAPACITY, "stream position overflows");
@@ -459,7 +465,19 @@ extern "C" int mrg8_server_progress(mrg8_server_t s, mrg8_completion* done, uint
       c.submit_time_s = r->t_submit;
       c.complete_time_s = r->t_complete;
     }
-    if (*ndone > 0 || wait_ms <= 0 || wall_now() >= deadline) return MRG8_OK;
+    if (*ndone > 0) return MRG8_OK;
+    if (s->inflight.empty() && s->active.empty() && !s->queued.empty() && s->done_unreported.empty()) {
+      // admit() left the head queued with nothing on the device: only
+      // completed requests that the caller still holds can occupy the arena
+      // and the slots, and no call but mrg8_server_release frees them
+      const Request* h = s->queued.front();
+      return fail(MRG8_ERR_BLOCKED,
+                  "request %llu (%zu bytes) cannot be admitted: %zu of %llu arena bytes are held by completed "
+                  "requests that have not been released",
+                  (unsigned long long)h->id, granule_bytes(h->bytes), s->arena.used(),
+                  (unsigned long long)s->cfg.max_output_bytes);
+    }
+    if (wait_ms <= 0 || wall_now() >= deadline) return MRG8_OK;
     if (s->inflight.empty() && s->active.empty() && s->queued.empty()) return MRG8_OK;
     std::this_thread::sleep_for(std::chrono::microseconds(20));
   }

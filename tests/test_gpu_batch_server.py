@@ -456,3 +456,39 @@ def test_server_replay_small_trace(gpu, oracle_mod):
     # the replay consumed the streams' positions in trace order
     for s in range(32):
         assert srv.get_stream(s)[1] == int(sizes[streams == s].sum())
+
+
+def test_server_cap_granule_and_blocked_queue(gpu, oracle_mod):
+    """The cap counts whole 256-byte granules (a 1000-byte cap serves 768
+    bytes), and a queue head that only a release can admit is reported as
+    MRG8_ERR_BLOCKED instead of a silent stall."""
+    from mrg8_amd.capi import ERR_BLOCKED, ERR_CAPACITY, Mrg8Error
+
+    say("server-granule-blocked")
+    srv = gpu.Server(num_streams=2, max_output_bytes=1000, base_seed=1)
+    with pytest.raises(Mrg8Error) as e:
+        srv.submit(0, 97, 0)  # 776 bytes take 1024 in granules
+    assert e.value.code == ERR_CAPACITY
+    srv.submit(0, 96, 0)  # 768 bytes fit exactly
+    drain(srv, 1, lambda c: None)
+    srv.close()
+
+    srv = gpu.Server(num_streams=2, max_output_bytes=4096, base_seed=1)
+    a = srv.submit(0, 300, 0)  # 2560 bytes in granules
+    got = []
+    while not got:
+        got = srv.progress(max_done=4, wait_ms=50)
+    assert got[0]["request_id"] == a  # completed, not released: holds its 2560 bytes
+    b = srv.submit(1, 300, 0)
+    with pytest.raises(Mrg8Error) as e:
+        srv.progress(max_done=4, wait_ms=50)
+    assert e.value.code == ERR_BLOCKED
+    srv.release(a)
+    done = []
+    while not done:
+        done = srv.progress(max_done=4, wait_ms=50)
+    assert done[0]["request_id"] == b
+    ok, _ = check_request(gpu, oracle_mod, done[0], oracle_mod.seed_state(2), 0)
+    assert ok
+    srv.release(b)
+    srv.close()
