@@ -61,6 +61,9 @@ def lib() -> ctypes.CDLL:
         L.mrg8o_stream_wsums.argtypes = [u32p, u32p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                          ctypes.c_uint64, ctypes.c_void_p]
         L.mrg8o_stream_wsums.restype = None
+        L.mrg8o_stream_windows.argtypes = [u32p, u32p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32,
+                                           ctypes.c_int, ctypes.c_void_p]
+        L.mrg8o_stream_windows.restype = None
         L.mrg8o_vec_outer_tp.argtypes = [u32p, u32p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int]
         L.mrg8o_vec_outer_tp.restype = ctypes.c_int
         L.mrg8o_vec_has_avx512.restype = ctypes.c_int
@@ -190,4 +193,15 @@ def stream_wsums(states, offsets, counts, formats) -> np.ndarray:
     out = np.zeros(len(cnt), dtype=np.uint64)
     lib().mrg8o_stream_wsums(_ptr(jump_table()), _ptr(st), off.ctypes.data, cnt.ctypes.data, fmt.ctypes.data,
                              len(cnt), out.ctypes.data)
+    return out
+
+
+def stream_windows(states, offsets, m: int, fmt: int) -> np.ndarray:
+    """(n, m) outputs: state k jumped by offsets[k], then its next m outputs in format fmt."""
+    st = np.ascontiguousarray(np.asarray(states, dtype=np.uint32).reshape(-1, 8))
+    off = np.ascontiguousarray(np.asarray(offsets, dtype=np.uint64))
+    assert off.size == st.shape[0]
+    out = np.empty((st.shape[0], m), dtype=np.uint32 if fmt == FMT_U32 else np.float64)
+    lib().mrg8o_stream_windows(_ptr(jump_table()), _ptr(st), off.ctypes.data, st.shape[0], m, fmt,
+                               out.ctypes.data)
     return out

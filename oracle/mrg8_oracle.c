@@ -255,3 +255,18 @@ void mrg8o_stream_wsums(const uint32_t* tab, const uint32_t* states, const uint6
     sums[k] = total;
   }
 }
+
+void mrg8o_stream_windows(const uint32_t* tab, const uint32_t* states, const uint64_t* offsets, uint64_t n,
+                          uint32_t m, int format, void* out) {
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int64_t k = 0; k < (int64_t)n; ++k) {
+    uint32_t st[8];
+    mrg8o_jump_vec(tab, offsets[k], states + 8 * k, st);
+    if (format == 2)
+      mrg8o_randint(st, (uint32_t*)out + (size_t)k * m, m);
+    else if (format == 0)
+      mrg8o_rand(st, (double*)out + (size_t)k * m, m);
+    else
+      mrg8o_rand01(st, (double*)out + (size_t)k * m, m);
+  }
+}

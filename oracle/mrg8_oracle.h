@@ -74,6 +74,13 @@ uint64_t mrg8o_wsum(const void* buf, uint64_t n, int esz);
 void mrg8o_stream_wsums(const uint32_t* tab, const uint32_t* states, const uint64_t* offsets,
                         const uint64_t* counts, const int32_t* formats, uint64_t n, uint64_t* sums);
 
+/* For each of n streams: jump states[8k..8k+7] by offsets[k] and write the
+ * next k outputs in `format` to out[k*m .. k*m + m) (esz 8 for doubles, 4 for
+ * words; OpenMP over streams).  The windows at given positions of many
+ * streams, for checking served requests at their ends. */
+void mrg8o_stream_windows(const uint32_t* tab, const uint32_t* states, const uint64_t* offsets, uint64_t n,
+                          uint32_t m, int format, void* out);
+
 /* CPU baselines (timed beside the GPU; reported, not optimised). */
 /* Vector + OpenMP restatement of knl/mrg8_vec.h mrg8_vec_outer_tp; writes
  * (x-1)/p doubles, advances st by n.  Returns the number of threads used. */

@@ -167,3 +167,27 @@ def test_advance_argument_errors(gpu):
     assert e.value.code == lib.ERR_ARG
     gpu.advance_states(d, d, 1, d, 0)  # n = 0 is a no-op
     d.free()
+
+
+def test_advance_p_words_on_the_centred_path(gpu, oracle_mod):
+    """Waves whose eight states have no zero word take the centred fast path;
+    put words equal to p (the reference's representation of residue 0) there:
+    all-p states, mixes of p and ordinary words, and single p words, eight
+    states per wave (8 lanes per state, 64 per wave) with no zero word."""
+    rng = np.random.default_rng(17)
+    n = 32  # four waves of eight states
+    states = rng.integers(1, P, size=(n, 8), dtype=np.int64).astype(np.uint32)
+    states[0:8] = P  # wave 0: every word p
+    for i in range(8, 16):  # wave 1: half the words p
+        states[i, rng.permutation(8)[:4]] = P
+    for i in range(16, 24):  # wave 2: one p word each, at every position
+        states[i, i - 16] = P
+    assert (states != 0).all()  # wave 3 (24..31): ordinary words only
+    for words, bits in ((1, 64), (4, 200)):
+        offs = [int.from_bytes(rng.bytes(32), "little") & ((1 << bits) - 1) for _ in range(n)]
+        offs[0], offs[9], offs[17] = 1, 255, (1 << 64) - 1 if words == 1 else (1 << 199) + 3
+        say("advance p words on the centred path", words)
+        got = run_advance(gpu, states, offs, words)
+        for i in range(n):
+            want = oracle_mod.jump_bits(states[i], set_bits(offs[i]))
+            assert got[i].tolist() == want.tolist(), (words, i, states[i].tolist(), hex(offs[i]))

@@ -6,8 +6,6 @@ import os
 import numpy as np
 import pytest
 
-from conftest import wsum
-
 pytestmark = pytest.mark.gpu
 
 SEED = 13579  # the reference's test seed (cuda/tests/test_mrg8_outer.cu:35)
@@ -328,9 +326,10 @@ def test_host_forms_many_chunks(gpu, oracle_mod):
     assert np.array_equal(g.get_state(), st)
 
 
-def test_c2_sampled_and_wsum(gpu, oracle_mod):
+def test_c2_sampled_and_hash(gpu, oracle_mod):
     """2^28 outputs in u32 and f64: 1e4 sampled indices against jump+1 output,
-    and a position-weighted checksum of the whole sequence."""
+    and the FNV-1a hash of the whole sequence against the oracle's and the
+    compiled reference's."""
     from mrg8_amd.hip import DeviceBuffer
 
     n = 1 << 28

@@ -380,3 +380,65 @@ def test_vector_baseline_equals_serial(oracle_mod, n, threads):
     want, _ = oracle_mod.generate(st, n, 1)
     assert used >= 1
     assert np.array_equal(got.view(np.uint64), want.view(np.uint64))
+
+
+# ---- the checker's checksums -------------------------------------------------
+# oracle.wsum / oracle.stream_wsums decide the full-size batched and served
+# checks; they are pinned here by closed forms and against an independent
+# big-integer evaluation of their definition sum_i w_i (2i+1) mod 2^64.
+
+def _wsum_bigint(arr):
+    w = arr.view(np.uint64) if arr.dtype == np.float64 else arr.astype(np.uint64)
+    return sum(int(x) * (2 * i + 1) for i, x in enumerate(w.tolist())) % (1 << 64)
+
+
+def test_wsum_closed_forms(oracle_mod):
+    from conftest import wsum as np_wsum
+
+    for n in (0, 1, 2, 7, 1000, 4097):
+        ones = np.ones(n, dtype=np.uint32)
+        assert oracle_mod.wsum(ones) == n * n  # sum of the first n odd numbers
+        assert np_wsum(ones) == n * n
+    # one non-zero word at position i weighs 2i + 1; f64 enters by its bit pattern
+    z = np.zeros(50, dtype=np.uint32)
+    z[17] = 0xFFFFFFFF
+    assert oracle_mod.wsum(z) == 0xFFFFFFFF * 35
+    d = np.zeros(9, dtype=np.float64)
+    d[4] = 1.0
+    assert oracle_mod.wsum(d) == (0x3FF0000000000000 * 9) % (1 << 64)
+    d[4] = -2.0  # the sign bit counts: 0xC000000000000000 * 9 wraps mod 2^64
+    assert oracle_mod.wsum(d) == (0xC000000000000000 * 9) % (1 << 64)
+
+
+def test_wsum_equals_definition(oracle_mod):
+    from conftest import wsum as np_wsum
+
+    s0 = oracle_mod.seed_state(4242)
+    for fmt in (0, 1, 2):
+        for n in (1, 63, 4096, 4099):
+            out, _ = oracle_mod.generate(s0, n, fmt)
+            want = _wsum_bigint(out)
+            assert oracle_mod.wsum(out) == want, (fmt, n)
+            assert np_wsum(out) == want, (fmt, n)
+
+
+def test_stream_wsums_and_windows_equal_generated(oracle_mod):
+    """Per-stream checksums and windows at non-zero offsets equal those of the
+    serial generator jumped there (counts across the 4096-output block edge)."""
+    rng = np.random.default_rng(99)
+    k = 12
+    states = rng.integers(1, 2**31 - 1, size=(k, 8), dtype=np.int64).astype(np.uint32)
+    offsets = rng.integers(0, 2**40, size=k, dtype=np.uint64)
+    offsets[0] = 0
+    counts = np.array([1, 64, 4095, 4096, 4097, 9000, 3, 5000, 8192, 100, 257, 4100], dtype=np.uint64)
+    fmts = np.array([0, 1, 2] * 4, dtype=np.int32)
+    sums = oracle_mod.stream_wsums(states, offsets, counts, fmts)
+    for i in range(k):
+        st = oracle_mod.jump(states[i], int(offsets[i]))
+        out, _ = oracle_mod.generate(st, int(counts[i]), int(fmts[i]))
+        assert int(sums[i]) == _wsum_bigint(out), i
+    for fmt in (0, 1, 2):
+        win = oracle_mod.stream_windows(states, offsets, 40, fmt)
+        for i in range(k):
+            out, _ = oracle_mod.generate(oracle_mod.jump(states[i], int(offsets[i])), 40, fmt)
+            assert win[i].tobytes() == out.tobytes(), (fmt, i)
