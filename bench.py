@@ -95,6 +95,51 @@ def valu_busy_from_profiles(name, kernel):
     return None
 
 
+def timed_series(step, stream, warmup, steps):
+    """W untimed launches, then K launches with a HIP event between every two
+    (on the launch stream): wall seconds, mean ms per launch and the K
+    per-launch durations."""
+    from mrg8_amd import hip
+
+    for _ in range(warmup):
+        step()
+    stream.synchronize()
+    hip.synchronize()
+    ev = [hip.Event() for _ in range(steps + 1)]
+    t0 = time.perf_counter()
+    ev[0].record(stream)
+    for i in range(steps):
+        step()
+        ev[i + 1].record(stream)
+    ev[-1].synchronize()
+    hip.synchronize()
+    wall = time.perf_counter() - t0
+    per = [ev[i].elapsed_ms(ev[i + 1]) for i in range(steps)]
+    total = ev[0].elapsed_ms(ev[-1])
+    for e in ev:
+        e.destroy()
+    return wall, total / steps, per
+
+
+def write_ceiling(buf, nbytes, s, reps=10):
+    """Store-bandwidth ceiling over the same buffer: this project's probe
+    (d[i] = i, the counterpart of cuda/stream.cu) with the fill's 1 KiB
+    non-temporal wave stores and in the reference's one-element-per-thread
+    form, and the runtime's own fill kernel (hipMemsetAsync)."""
+    from mrg8_amd import hip, probe
+
+    n = nbytes // 8
+    out = {}
+    for name, fn in (("probe_fill_pattern", lambda: probe.write(buf.ptr, n, 1, 0, s)),
+                     ("probe_reference_form", lambda: probe.write(buf.ptr, n, 0, 0, s)),
+                     ("memset", lambda: hip.memset_async(buf.ptr, 0, nbytes, s))):
+        fn()
+        km = time_launches(fn, s, reps)
+        out[name] = {"ms": round(km, 4), "gbs": round(nbytes / (km * 1e-3) / 1e9, 1)}
+    log("write ceiling: " + ", ".join(f"{k} {v['gbs']:.0f} GB/s" for k, v in out.items()))
+    return out
+
+
 def bench_single(args):
     import mrg8_amd
     from mrg8_amd import hip
@@ -102,27 +147,13 @@ def bench_single(args):
     s = hip.Stream()
     buf = hip.DeviceBuffer(N_SINGLE * 8)
     g = mrg8_amd.mrg8_cuda(SEED)
-    out = {}
+    out = {"ceiling": write_ceiling(buf, N_SINGLE * 8, s)}
     for fmt, name, esz in ((mrg8_amd.F64, "f64", 8), (mrg8_amd.U32, "u32", 4)):
         step = lambda: g.generate(buf, N_SINGLE, fmt, stream=s)  # noqa: E731
-        for _ in range(args.warmup):
-            step()
-        s.synchronize()
-        hip.synchronize()
-        t0 = time.perf_counter()
-        e0, e1 = hip.Event(), hip.Event()
-        e0.record(s)
-        for _ in range(args.steps):
-            step()
-        e1.record(s)
-        e1.synchronize()
-        hip.synchronize()
-        wall = time.perf_counter() - t0
-        ev_s = e0.elapsed_ms(e1) / 1e3
-        # mean launch duration: HIP events bracketing the timed region on the launch stream
-        out[name] = {"wall_s": wall, "event_s": ev_s, "kernel_ms": ev_s * 1e3 / args.steps, "bytes": N_SINGLE * esz}
-        log(f"single {name}: {N_SINGLE * args.steps / wall / 1e9:.1f} G samples/s wall, "
-            f"kernel {out[name]['kernel_ms']:.3f} ms")
+        wall, km, per = timed_series(step, s, args.warmup, args.steps)
+        out[name] = {"wall_s": wall, "kernel_ms": km, "per_launch_ms": per, "bytes": N_SINGLE * esz}
+        log(f"single {name}: {N_SINGLE * args.steps / wall / 1e9:.1f} G samples/s wall, kernel {km:.3f} ms "
+            f"(launches {min(per):.3f}-{max(per):.3f} ms)")
     # the reference's full-compare size
     small = 1 << 24
     step = lambda: g.generate(buf, small, mrg8_amd.F64, stream=s)  # noqa: E731
@@ -131,16 +162,42 @@ def bench_single(args):
     km = time_launches(step, s, 20)
     out["small"] = {"kernel_ms": km, "gsps": small / (km * 1e-3) / 1e9,
                     "gbs": small * 8 / (km * 1e-3) / 1e9}
-    # measured write ceiling: the runtime's own fill kernel over the same 2 GiB
-    # (the counterpart of the reference's stream benchmark, cuda/stream.cu)
-    fill = lambda: hip.memset_async(buf.ptr, 0, N_SINGLE * 8, s)  # noqa: E731
-    fill()
-    km = time_launches(fill, s, 10)
-    out["write_ceiling"] = {"kernel": "hipMemsetAsync 2 GiB", "ms": km, "gbs": N_SINGLE * 8 / (km * 1e-3) / 1e9}
-    log(f"write ceiling (memset): {out['write_ceiling']['gbs']:.0f} GB/s")
+    out["clock"] = clock_series(g, buf, s)
     buf.free()
     s.destroy()
     return out
+
+
+def clock_series(g, buf, s, idle_s=2.0, launches=60):
+    """The 2^28-double fill from an idle GPU, each launch followed by a
+    shader-clock sample: how the power manager's clock follows a sustained
+    fill (the timed window above starts the same way)."""
+    import mrg8_amd
+    from mrg8_amd import hip, probe
+
+    clk = probe.ClockProbe(launches)
+    ev = [hip.Event() for _ in range(2 * launches)]
+    s.synchronize()
+    time.sleep(idle_s)
+    for i in range(launches):
+        ev[2 * i].record(s)
+        g.generate(buf, N_SINGLE, mrg8_amd.F64, stream=s)
+        ev[2 * i + 1].record(s)
+        clk.sample(s)
+    s.synchronize()
+    us = [round(ev[2 * i].elapsed_ms(ev[2 * i + 1]) * 1e3, 1) for i in range(launches)]
+    mhz = clk.mhz()
+    clk.free()
+    for e in ev:
+        e.destroy()
+    tail = us[-15:]
+    steady = sum(tail) / len(tail) * 1e-3
+    log(f"clock series: {min(mhz):.0f}-{max(mhz):.0f} MHz, fill {min(us):.0f}-{max(us):.0f} us, "
+        f"last 15 launches {steady:.3f} ms")
+    return {"idle_before_s": idle_s, "kernel_us": us, "sclk_mhz": mhz,
+            "last15_kernel_ms": round(steady, 4),
+            "last15_frac": round(N_SINGLE * 8 / (steady * 1e-3) / 1e9 / PEAK_GBS, 4),
+            "last15_sclk_mhz": round(sum(mhz[-15:]) / 15, 1)}
 
 
 def bench_host_form(n=1 << 26, reps=3):
@@ -415,6 +472,7 @@ def main():
     value = N_SINGLE * args.steps / f64["wall_s"] / 1e9
     achieved = f64["bytes"] / (f64["kernel_ms"] * 1e-3) / 1e9
     traffic = traffic_from_profiles("fill_f64_2e28_counters.txt")
+    ceiling = single["ceiling"]["probe_fill_pattern"]
     line = {
         "metric": METRIC, "value": round(value, 2), "unit": "G samples/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(f64["wall_s"] / args.steps * 1e3, 4), "higher_is_better": True,
@@ -426,16 +484,28 @@ def main():
                      "frac": round(achieved / PEAK_GBS, 4),
                      "traffic": traffic,
                      "kernel": "k_fill<F64,4>", "kernel_ms": round(f64["kernel_ms"], 4),
+                     "per_launch_ms": [round(x, 4) for x in f64["per_launch_ms"]],
                      "valu_busy_pct": valu_busy_from_profiles("fill_f64_2e28_counters.txt", "k_fill<F64,4>"),
                      "algorithmic_bytes_per_launch": f64["bytes"]},
         "u32": {"gsps": round(N_SINGLE * args.steps / single["u32"]["wall_s"] / 1e9, 2),
                 "kernel_ms": round(single["u32"]["kernel_ms"], 4),
-                "gbs": round(single["u32"]["bytes"] / (single["u32"]["kernel_ms"] * 1e-3) / 1e9, 1)},
+                "gbs": round(single["u32"]["bytes"] / (single["u32"]["kernel_ms"] * 1e-3) / 1e9, 1),
+                "frac": round(single["u32"]["bytes"] / (single["u32"]["kernel_ms"] * 1e-3) / 1e9 / PEAK_GBS, 4),
+                "per_launch_ms": [round(x, 4) for x in single["u32"]["per_launch_ms"]]},
         "small_2e24_f64": {k: round(v, 4) for k, v in single["small"].items()},
-        "write_ceiling": {"kernel": single["write_ceiling"]["kernel"],
-                          "gbs": round(single["write_ceiling"]["gbs"], 1),
-                          "frac": round(single["write_ceiling"]["gbs"] / PEAK_GBS, 4),
-                          "fill_frac_of_ceiling": round(achieved / single["write_ceiling"]["gbs"], 4)},
+        "write_ceiling": {"kernel": "mrg8_probe_write: d[i] = i over the same 2 GiB with the fill's 1 KiB "
+                                    "non-temporal wave stores (the counterpart of cuda/stream.cu), measured "
+                                    "just before the float64 leg",
+                          "gbs": ceiling["gbs"], "frac": round(ceiling["gbs"] / PEAK_GBS, 4),
+                          "fill_frac_of_ceiling": round(achieved / ceiling["gbs"], 4),
+                          "reference_form_gbs": single["ceiling"]["probe_reference_form"]["gbs"],
+                          "memset_gbs": single["ceiling"]["memset"]["gbs"]},
+        "clock": dict(single["clock"], note=(
+            "shader clock (s_memtime against the 100 MHz s_memrealtime, sampled after each launch) of a "
+            "separate 60-launch series of the same fill started after 2 s idle: the power manager lowers "
+            "the clock a few launches into a sustained fill and restores it over the next tens of launches; "
+            "the timed window above (launches warmup+1..warmup+steps) falls in that span when warmup is small, "
+            "and last15_* is the same fill once the clock has recovered")),
     }
     if not args.quick:
         line["batched"] = bench_batched(args)

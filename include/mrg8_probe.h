@@ -1,0 +1,31 @@
+/*
+ * mrg8_probe.h -- measurement kernels used by the benchmark (libmrg8_probe.so).
+ * Not part of the generator's interface.  Status: 0 ok, -1 invalid argument,
+ * -2 launch failed.  Pointers are device memory; `stream` is a hipStream_t.
+ */
+#ifndef MRG8_PROBE_H
+#define MRG8_PROBE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* d[i] = (double)i for i < n: the write-bandwidth calibration of the
+ * reference (cuda/stream.cu:8-15, one element per thread in 512-thread
+ * blocks) with mode 0; with mode 1 the generator fill's store pattern
+ * (runs of `tile` doubles per wave, 0 = 2048, a multiple of 128; 1 KiB
+ * non-temporal wave stores; d 1 KiB-aligned) and no arithmetic. */
+int mrg8_probe_write(void* d, uint64_t n, int mode, uint64_t tile, void* stream);
+
+/* One wave spins ~20 us and writes d_out[0] = shader-clock ticks
+ * (s_memtime) and d_out[1] = 100 MHz reference ticks (s_memrealtime) over
+ * the same interval: clock in MHz = 100 * d_out[0] / d_out[1]. */
+int mrg8_probe_sclk(uint64_t* d_out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MRG8_PROBE_H */

@@ -1,4 +1,5 @@
-"""Builds libmrg8_amd.so (gfx950 kernels + C ABI) in-tree with hipcc.
+"""Builds libmrg8_amd.so (gfx950 kernels + C ABI) and libmrg8_probe.so (the
+benchmark's measurement kernels) in-tree with hipcc.
 
     python -m mrg8_amd.build            # compile (always rebuilds)
     python -m mrg8_amd.build --check    # print the command lines only
@@ -16,6 +17,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 ROOT = os.path.dirname(HERE)
 OUT = os.path.join(HERE, "libmrg8_amd.so")
+PROBE_SRC = os.path.join(CSRC, "probe", "mrg8_probe.hip")
+PROBE_OUT = os.path.join(HERE, "libmrg8_probe.so")
 OBJDIR = os.path.join(ROOT, "_build", "mrg8_amd")
 ARCH = "gfx950"
 
@@ -56,6 +59,10 @@ def link_cmd(objs):
     return [hipcc(), "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", OUT + ".tmp"] + objs
 
 
+def probe_cmd():
+    return [hipcc()] + COMMON + DEVICE + ["-shared", "-o", PROBE_OUT + ".tmp", PROBE_SRC]
+
+
 def build(verbose: bool = False) -> str:
     os.makedirs(OBJDIR, exist_ok=True)
     cmds = compile_cmds()
@@ -68,12 +75,15 @@ def build(verbose: bool = False) -> str:
             raise RuntimeError("compile failed:\n" + " ".join(cmd) + "\n" + r.stderr)
         return err
 
-    with cf.ThreadPoolExecutor(max_workers=min(8, len(cmds))) as ex:
+    with cf.ThreadPoolExecutor(max_workers=min(8, len(cmds) + 1)) as ex:
+        probe = ex.submit(run, probe_cmd())
         for err in ex.map(run, [c for _, c in cmds]):
             if err.strip() and verbose:
                 print(err, file=sys.stderr)
+        probe.result()
     run(link_cmd([o for o, _ in cmds]))
     os.replace(OUT + ".tmp", OUT)
+    os.replace(PROBE_OUT + ".tmp", PROBE_OUT)
     return OUT
 
 
@@ -86,6 +96,7 @@ def main():
         for _, c in compile_cmds():
             print(" ".join(c))
         print(" ".join(link_cmd(["<objs>"])))
+        print(" ".join(probe_cmd()))
         return
     print(build(verbose=a.verbose))
 

@@ -1,0 +1,86 @@
+// Measurement kernels for the benchmark (libmrg8_probe.so, include/mrg8_probe.h).
+//
+//  * Write probe: d_a[i] = i over n doubles, the counterpart of the
+//    reference's bandwidth calibration (cuda/stream.cu:8-15,42-56).  Mode 0
+//    keeps its shape (one element per thread, 512-thread blocks, plain
+//    stores); mode 1 writes with the store pattern of the generator's fill
+//    kernel and no arithmetic (each wave one run of `tile` doubles, 16 bytes
+//    per lane and 1 KiB per wave store, non-temporal).
+//  * Shader-clock probe: one wave counts s_memtime (shader clock) ticks
+//    against the constant 100 MHz s_memrealtime over about 20 us, so a
+//    caller can see the clock the power manager has set between launches.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "../../../include/mrg8_probe.h"
+
+namespace {
+
+typedef double d2_t __attribute__((ext_vector_type(2)));
+
+__global__ __launch_bounds__(512) void k_write_elem(double* __restrict__ d, uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) d[i] = (double)i;
+}
+
+// Wave w writes elements [w * tile, min((w + 1) * tile, n)); tile is a
+// multiple of 128 and d is 1 KiB-aligned (checked by the caller), so every
+// full wave store covers 1 KiB of whole cache lines.
+__global__ __launch_bounds__(1024) void k_write_tiles(double* __restrict__ d, uint64_t n, uint64_t tile) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint64_t w = (uint64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const uint64_t first = w * tile;
+  if (first >= n) return;
+  const uint64_t end = first + tile < n ? first + tile : n;
+  uint64_t i = first;
+  for (; i + 128 <= end; i += 128) {
+    const uint64_t k = i + 2 * lane;
+    d2_t v = {(double)k, (double)(k + 1)};
+    __builtin_nontemporal_store(v, reinterpret_cast<d2_t*>(d + k));
+  }
+  for (uint64_t k = i + lane; k < end; k += 64) __builtin_nontemporal_store((double)k, d + k);
+}
+
+__global__ void k_sclk(uint64_t* __restrict__ out) {
+  const uint64_t r0 = __builtin_amdgcn_s_memrealtime();
+  const uint64_t t0 = __builtin_amdgcn_s_memtime();
+  uint64_t r1 = r0;
+  // about 20 us of the 100 MHz counter; the iteration bound ends the loop
+  // even if that counter did not advance
+  for (int i = 0; i < (1 << 22) && r1 - r0 < 2000; ++i) r1 = __builtin_amdgcn_s_memrealtime();
+  const uint64_t t1 = __builtin_amdgcn_s_memtime();
+  if (threadIdx.x == 0) {
+    out[0] = t1 - t0;
+    out[1] = r1 - r0;
+  }
+}
+
+}  // namespace
+
+extern "C" int mrg8_probe_write(void* d, uint64_t n, int mode, uint64_t tile, void* stream) {
+  if (!d && n) return -1;
+  if (n == 0) return 0;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (mode == 0) {
+    const uint64_t blocks = (n + 511) / 512;
+    if (blocks > 0xffffffffull) return -1;
+    hipLaunchKernelGGL(k_write_elem, dim3((uint32_t)blocks), dim3(512), 0, s, static_cast<double*>(d), n);
+  } else if (mode == 1) {
+    if (tile == 0) tile = 2048;
+    if (tile % 128 || (reinterpret_cast<uintptr_t>(d) & 1023)) return -1;
+    const uint64_t waves = (n + tile - 1) / tile;
+    const uint64_t blocks = (waves + 15) / 16;
+    if (blocks > 0xffffffffull) return -1;
+    hipLaunchKernelGGL(k_write_tiles, dim3((uint32_t)blocks), dim3(1024), 0, s, static_cast<double*>(d), n, tile);
+  } else {
+    return -1;
+  }
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+extern "C" int mrg8_probe_sclk(uint64_t* d_out, void* stream) {
+  if (!d_out) return -1;
+  hipLaunchKernelGGL(k_sclk, dim3(1), dim3(64), 0, static_cast<hipStream_t>(stream), d_out);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}

@@ -1,0 +1,64 @@
+"""ctypes binding to libmrg8_probe.so: the benchmark's write-bandwidth probe
+(the counterpart of the reference's cuda/stream.cu) and shader-clock probe
+(include/mrg8_probe.h).  Measurement only; not part of the generator."""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmrg8_probe.so")
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} missing: build with `python -m mrg8_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        L.mrg8_probe_write.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_uint64,
+                                       ctypes.c_void_p]
+        L.mrg8_probe_sclk.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        _lib = L
+    return _lib
+
+
+def _s(stream):
+    return ctypes.c_void_p(stream.handle if hasattr(stream, "handle") else stream)
+
+
+def write(dptr: int, n: int, mode: int = 1, tile: int = 0, stream=None) -> None:
+    """d[i] = i for n doubles at device address dptr (mode 0: one element per
+    thread as cuda/stream.cu; mode 1: the fill's 1 KiB non-temporal wave stores)."""
+    rc = lib().mrg8_probe_write(ctypes.c_void_p(int(dptr)), n, mode, tile, _s(stream))
+    if rc != 0:
+        raise RuntimeError(f"mrg8_probe_write failed ({rc})")
+
+
+class ClockProbe:
+    """Shader clock (MHz) sampled by a one-wave kernel queued on a stream."""
+
+    def __init__(self, capacity: int):
+        from . import hip
+
+        self.capacity = capacity
+        self.buf = hip.DeviceBuffer(16 * capacity)
+        self.n = 0
+
+    def sample(self, stream) -> None:
+        if self.n >= self.capacity:
+            raise IndexError("clock probe buffer full")
+        rc = lib().mrg8_probe_sclk(ctypes.c_void_p(self.buf.ptr + 16 * self.n), _s(stream))
+        if rc != 0:
+            raise RuntimeError(f"mrg8_probe_sclk failed ({rc})")
+        self.n += 1
+
+    def mhz(self) -> list:
+        v = self.buf.copy_to_host(np.uint64, 2 * self.n).reshape(-1, 2)
+        return [round(100.0 * float(t) / float(r), 1) if r else 0.0 for t, r in v]
+
+    def free(self) -> None:
+        self.buf.free()
