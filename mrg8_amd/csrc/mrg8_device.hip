@@ -375,10 +375,13 @@ __device__ __forceinline__ void generate_run(typename Out<FMT>::T* __restrict__ 
     dot_residues<PL>(k, e, r);
     store_lane<FMT, PL>(base, lane, r);
     base += kPer;
-    // next state: the last eight words, newest first
+    // next state: the last eight words, newest first (not needed after the
+    // run's last iteration: 8 of 16 readlanes of a 256-output tile at 2 per lane)
+    if (it + 1 < full || rem) {
 #pragma unroll
-    for (int j = 0; j < 8; ++j)
-      s[j] = __builtin_amdgcn_readlane(r[L::slot_of(kPer - 1 - j)], L::lane_of(kPer - 1 - j)) + 1u;
+      for (int j = 0; j < 8; ++j)
+        s[j] = __builtin_amdgcn_readlane(r[L::slot_of(kPer - 1 - j)], L::lane_of(kPer - 1 - j)) + 1u;
+    }
   }
   if (rem) {
     int32_t e[8];
