@@ -255,6 +255,25 @@ int mrg8_batch_use_graph(mrg8_batch_t b, int enable);
  * is a text file of results keyed by device and workload class. */
 int mrg8_batch_autotune(mrg8_batch_t b, void* stream, const char* cache_path, mrg8_batch_config* best);
 
+/* Batched generation that allocates nothing.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * same generation as mrg8_batch_set + mrg8_batch_run with configuration
+ * `cfg` (NULL: the batch default, 1024 threads and one 2048-output tile per
+ * wave): identical outputs, stream-ordered on `stream`.  The batch block
+ * (segments, tile prefix sums, tile map) goes to a fixed device scratch area
+ * of MRG8_NOALLOC_SCRATCH_BYTES that the library holds on every device it
+ * runs on until the process ends; the call makes no hipMalloc/hipFree and no
+ * heap allocation.  mrg8_init_device(device) must have been called
+ * (MRG8_ERR_STATE otherwise: creating the device's tables allocates).  Work
+ * whose block exceeds the area (segments x 64 B + tiles x 4 B, about) is
+ * MRG8_ERR_CAPACITY.  All calls on a device share its scratch area: calls
+ * that overlap on one device race, so issue them on one stream or wait in
+ * between.  `descs` may be reused as soon as the call returns. */
+#define MRG8_NOALLOC_SCRATCH_BYTES (4u << 20)
+int mrg8_batch_generate_noalloc(int device, const mrg8_stream_desc* descs, uint32_t count,
+                                const mrg8_batch_config* cfg, void* stream);
+
 /* ---- serving: resident streams, continuous batching -------------------- */
 
 typedef struct mrg8_server_config {

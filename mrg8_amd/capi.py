@@ -111,6 +111,8 @@ SIGNATURES = {
     "mrg8_batch_run": (ctypes.c_int, [H, vp]),
     "mrg8_batch_use_graph": (ctypes.c_int, [H, ctypes.c_int]),
     "mrg8_batch_autotune": (ctypes.c_int, [H, vp, ctypes.c_char_p, ctypes.POINTER(BatchConfig)]),
+    "mrg8_batch_generate_noalloc": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(StreamDesc), ctypes.c_uint32,
+                                                   ctypes.POINTER(BatchConfig), vp]),
     "mrg8_server_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ServerConfig), ctypes.c_uint32,
                                           ctypes.POINTER(H)]),
     "mrg8_server_destroy": (ctypes.c_int, [H]),

@@ -35,9 +35,9 @@ struct mrg8_batch {
   bool use_graph = false;
 };
 
-namespace {
+namespace mrg8 {
 
-int validate_config(const mrg8_batch_config& c) {
+int validate_batch_config(const mrg8_batch_config& c) {
   if (c.block_threads < 64 || c.block_threads > 1024 || c.block_threads % 64)
     return fail(MRG8_ERR_ARG, "block_threads %d must be a multiple of 64 in [64, 1024]", c.block_threads);
   if (c.blocks_per_cu < 0 || c.blocks_per_cu * (c.block_threads / 64) > 32)
@@ -47,6 +47,42 @@ int validate_config(const mrg8_batch_config& c) {
     return fail(MRG8_ERR_ARG, "tile_outputs must be a positive multiple of 256, at most 2^40");
   return MRG8_OK;
 }
+
+int check_descriptors(const mrg8_stream_desc* descs, uint32_t count, uint64_t tile, uint64_t* tiles,
+                      uint64_t* total) {
+  *tiles = *total = 0;
+  for (uint32_t i = 0; i < count; ++i) {
+    const mrg8_stream_desc& d = descs[i];
+    const int esz = format_size(d.format);
+    if (!esz) return fail(MRG8_ERR_ARG, "segment %u: unknown format %d", i, d.format);
+    if (d.reserved) return fail(MRG8_ERR_ARG, "segment %u: reserved field must be 0", i);
+    if (!state_valid(d.state)) return fail(MRG8_ERR_ARG, "segment %u: state words must be at most 2^31 - 1", i);
+    if (d.count && !d.out) return fail(MRG8_ERR_ARG, "segment %u: null output", i);
+    if (reinterpret_cast<uintptr_t>(d.out) % (uintptr_t)esz)
+      return fail(MRG8_ERR_ARG, "segment %u: output must be %d-byte aligned", i, esz);
+    if (d.count > UINT64_MAX - d.offset) return fail(MRG8_ERR_ARG, "segment %u: offset + count overflows", i);
+    *tiles += segment_tiles(d.out, d.format, d.count, tile);
+    *total += d.count;
+  }
+  return MRG8_OK;
+}
+
+void write_segments(BatchSegment* segs, const mrg8_stream_desc* descs, uint32_t count) {
+  for (uint32_t i = 0; i < count; ++i) {
+    const mrg8_stream_desc& d = descs[i];
+    BatchSegment& s = segs[i];
+    std::memcpy(s.state, d.state, sizeof s.state);
+    s.offset = d.offset;
+    s.count = d.count;
+    s.out = d.out;
+    s.format = d.format;
+    s.stream = kInlineState;
+  }
+}
+
+}  // namespace mrg8
+
+namespace {
 
 LaunchConfig launch_config(const mrg8_batch_config& c) {
   LaunchConfig l;
@@ -102,7 +138,7 @@ extern "C" int mrg8_batch_get_config(mrg8_batch_t b, mrg8_batch_config* cfg) {
 extern "C" int mrg8_batch_set_config(mrg8_batch_t b, const mrg8_batch_config* cfg) {
   clear_error();
   if (!b || !cfg) return fail(MRG8_ERR_ARG, "null argument");
-  int rc = validate_config(*cfg);
+  int rc = validate_batch_config(*cfg);
   if (rc) return rc;
   const uint32_t n = b->slot.staged_count();
   if (n) {
@@ -123,35 +159,13 @@ extern "C" int mrg8_batch_set(mrg8_batch_t b, const mrg8_stream_desc* descs, uin
   if (count > b->slot.capacity())
     return fail(MRG8_ERR_CAPACITY, "%u segments exceed max_streams %u", count, b->slot.capacity());
   uint64_t tiles = 0, total = 0;
-  for (uint32_t i = 0; i < count; ++i) {
-    const mrg8_stream_desc& d = descs[i];
-    const int esz = format_size(d.format);
-    if (!esz) return fail(MRG8_ERR_ARG, "segment %u: unknown format %d", i, d.format);
-    if (d.reserved) return fail(MRG8_ERR_ARG, "segment %u: reserved field must be 0", i);
-    if (!state_valid(d.state)) return fail(MRG8_ERR_ARG, "segment %u: state words must be at most 2^31 - 1", i);
-    if (d.count && !d.out) return fail(MRG8_ERR_ARG, "segment %u: null output", i);
-    if (reinterpret_cast<uintptr_t>(d.out) % (uintptr_t)esz)
-      return fail(MRG8_ERR_ARG, "segment %u: output must be %d-byte aligned", i, esz);
-    if (d.count > UINT64_MAX - d.offset) return fail(MRG8_ERR_ARG, "segment %u: offset + count overflows", i);
-    tiles += segment_tiles(d.out, d.format, d.count, b->cfg.tile_outputs);
-    total += d.count;
-  }
+  int rc = check_descriptors(descs, count, b->cfg.tile_outputs, &tiles, &total);
+  if (rc) return rc;
   if (tiles > b->max_tiles)
     return fail(MRG8_ERR_CAPACITY, "%llu tiles exceed max_tiles %llu", (unsigned long long)tiles,
                 (unsigned long long)b->max_tiles);
-  int rc = b->slot.wait_staging();  // the previous run may still read the staging block
-  if (rc) return rc;
-  BatchSegment* segs = b->slot.segs();
-  for (uint32_t i = 0; i < count; ++i) {
-    const mrg8_stream_desc& d = descs[i];
-    BatchSegment& s = segs[i];
-    std::memcpy(s.state, d.state, sizeof s.state);
-    s.offset = d.offset;
-    s.count = d.count;
-    s.out = d.out;
-    s.format = d.format;
-    s.stream = kInlineState;
-  }
+  if ((rc = b->slot.wait_staging()) != MRG8_OK) return rc;  // the previous run may still read the staging block
+  write_segments(b->slot.segs(), descs, count);
   b->slot.finalise(count, b->cfg.tile_outputs);
   b->total = total;
   return MRG8_OK;
@@ -196,7 +210,7 @@ bool cache_lookup(const char* path, const std::string& key, mrg8_batch_config* c
   double ms;
   bool found = false;
   while (f >> k >> c.block_threads >> c.blocks_per_cu >> c.tile_outputs >> ms)
-    if (k == key && validate_config(c) == MRG8_OK) {
+    if (k == key && validate_batch_config(c) == MRG8_OK) {
       *cfg = c;  // the last entry for a key wins
       found = true;
     }
@@ -247,7 +261,7 @@ extern "C" int mrg8_batch_autotune(mrg8_batch_t b, void* stream, const char* cac
       for (uint64_t t : tiles) {
         if (rc) break;
         const mrg8_batch_config c{bt, w / (bt / 64), t};
-        if ((w > 0 && c.blocks_per_cu < 1) || validate_config(c) != MRG8_OK) continue;
+        if ((w > 0 && c.blocks_per_cu < 1) || validate_batch_config(c) != MRG8_OK) continue;
         const uint64_t nt = count_tiles(b->slot.segs(), b->slot.staged_count(), t);
         if (nt > b->max_tiles) continue;
         if (w == 0 && (nt + bt / 64 - 1) / (bt / 64) * (uint64_t)bt > 0xffffffffull) continue;  // 32-bit grid

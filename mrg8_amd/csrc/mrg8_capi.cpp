@@ -118,14 +118,27 @@ int device_context(int device, DeviceContext** out) {
   e = hipMemcpy(ctx->tables.lane, lane.data(), sizeof(CoefRecord) * lane.size(), hipMemcpyHostToDevice);
   if (e == hipSuccess)
     e = hipMemcpy(ctx->tables.jump, jump.data(), sizeof(CoefRecord) * jump.size(), hipMemcpyHostToDevice);
-  (void)hipSetDevice(prev);
   if (e != hipSuccess) {
     (void)hipFree(p);
+    (void)hipSetDevice(prev);
     return hip_fail(e, "hipMemcpy(tables)");
+  }
+  // the fixed scratch area of mrg8_batch_generate_noalloc: its address on
+  // this device, taken once, with the device current
+  int rc = noalloc_scratch_address(&ctx->noalloc_scratch);
+  (void)hipSetDevice(prev);
+  if (rc) {
+    (void)hipFree(p);
+    return rc;
   }
   *out = ctx.get();
   g_devices[device] = std::move(ctx);
   return MRG8_OK;
+}
+
+DeviceContext* existing_device_context(int device) {
+  std::lock_guard<std::mutex> lk(g_dev_mu);
+  return device >= 0 && device < (int)g_devices.size() ? g_devices[device].get() : nullptr;
 }
 
 int ensure_wide_table(DeviceContext& ctx) {

@@ -8,6 +8,7 @@
 #include <cstdint>
 #include <mutex>
 
+#include "../../include/mrg8.h"
 #include "mrg8_device.hpp"
 
 namespace mrg8 {
@@ -26,6 +27,7 @@ struct DeviceContext {
   TunedShape tuned[3][64];  // [format][floor(log2 n)], set by mrg8_tune_fill
   std::mutex tuned_mu;      // guards `tuned` (fills on other handles read it)
   std::mutex wide_mu;       // guards the first-use build of tables.wide
+  uint8_t* noalloc_scratch = nullptr;  // this device's copy of the fixed scratch area
 };
 
 // Builds the table of byte positions kJumpPositions..kWidePositions-1 on the
@@ -39,6 +41,9 @@ void clear_error();
 
 const JumpTable& host_jump_table();
 int device_context(int device, DeviceContext** out);
+// The device's context if mrg8_init_device (or any call that needs it) has
+// created it, else null; creates nothing.
+DeviceContext* existing_device_context(int device);
 bool state_valid(const uint32_t st[8]);
 int format_size(int format);
 
@@ -47,6 +52,18 @@ int format_size(int format);
 // default.
 int fill_device(DeviceContext& ctx, const uint32_t state[8], uint64_t offset, void* dptr, uint64_t n, int format,
                 hipStream_t stream, int block_threads, uint64_t tile, int per_lane);
+
+// Batch descriptors (include/mrg8.h): checks a launch configuration; checks
+// `count` descriptors and sums their tiles (of `tile` outputs) and outputs;
+// converts them to the device segment layout (inline states).
+int validate_batch_config(const mrg8_batch_config& c);
+int check_descriptors(const mrg8_stream_desc* descs, uint32_t count, uint64_t tile, uint64_t* tiles,
+                      uint64_t* total);
+void write_segments(BatchSegment* segs, const mrg8_stream_desc* descs, uint32_t count);
+
+// Device address of the library's fixed scratch area on the current device
+// (taken once per device when its context is created).
+int noalloc_scratch_address(uint8_t** ptr);
 
 // RAII device switch.
 class DeviceGuard {

@@ -33,7 +33,6 @@ int BatchSlot::init(DeviceContext* ctx, uint32_t cap, uint64_t max_tiles, const 
     return rc;
   }
   std::memset(hmem_, 0, total);
-  h_hdr_ = reinterpret_cast<BatchHeader*>(hmem_);
   h_segs_ = reinterpret_cast<BatchSegment*>(hmem_ + sizeof(BatchHeader));
   return MRG8_OK;
 }
@@ -68,24 +67,36 @@ int BatchSlot::wait_staging() {
   return MRG8_OK;
 }
 
-uint64_t BatchSlot::finalise(uint32_t count, uint64_t tile) {
+size_t batch_block_bytes(uint32_t count, uint64_t ntiles) {
+  return tile_seg_offset(count) + sizeof(uint32_t) * (size_t)ntiles;
+}
+
+size_t layout_batch_block(uint8_t* mem, uint32_t count, uint64_t tile, uint64_t max_tiles, uint64_t* ntiles) {
   const size_t tb_off = tile_begin_offset(count), ts_off = tile_seg_offset(count);
-  uint64_t* tiles = reinterpret_cast<uint64_t*>(hmem_ + tb_off);
-  uint32_t* tile_seg = reinterpret_cast<uint32_t*>(hmem_ + ts_off);
+  BatchHeader* hdr = reinterpret_cast<BatchHeader*>(mem);
+  const BatchSegment* segs = reinterpret_cast<const BatchSegment*>(mem + sizeof(BatchHeader));
+  uint64_t* tiles = reinterpret_cast<uint64_t*>(mem + tb_off);
+  uint32_t* tile_seg = reinterpret_cast<uint32_t*>(mem + ts_off);
   uint64_t t = 0;
   for (uint32_t i = 0; i < count; ++i) {
     tiles[i] = t;
-    const uint64_t n = segment_tiles(h_segs_[i].out, h_segs_[i].format, h_segs_[i].count, tile);
-    const uint64_t end = std::min<uint64_t>(t + n, max_tiles_);
+    const uint64_t n = segment_tiles(segs[i].out, segs[i].format, segs[i].count, tile);
+    const uint64_t end = std::min<uint64_t>(t + n, max_tiles);
     if (t < end) std::fill(tile_seg + t, tile_seg + end, i);
     t += n;
   }
   tiles[count] = t;
-  h_hdr_->nsegs = count;
-  h_hdr_->ntiles = t;
-  h_hdr_->tile_begin_off = tb_off;
-  h_hdr_->tile_seg_off = ts_off;
-  used_bytes_ = ts_off + sizeof(uint32_t) * (size_t)std::min<uint64_t>(t, max_tiles_);
+  hdr->nsegs = count;
+  hdr->ntiles = t;
+  hdr->tile_begin_off = tb_off;
+  hdr->tile_seg_off = ts_off;
+  *ntiles = t;
+  return batch_block_bytes(count, std::min<uint64_t>(t, max_tiles));
+}
+
+uint64_t BatchSlot::finalise(uint32_t count, uint64_t tile) {
+  uint64_t t = 0;
+  used_bytes_ = layout_batch_block(hmem_, count, tile, max_tiles_, &t);
   count_ = count;
   ntiles_ = t;
   dirty_ = true;

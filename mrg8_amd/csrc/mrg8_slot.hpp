@@ -30,6 +30,14 @@ struct LaunchConfig {
   uint64_t tile = 4096;
 };
 
+// Bytes of a batch block with `count` segments and `ntiles` tile-map entries.
+size_t batch_block_bytes(uint32_t count, uint64_t ntiles);
+// Lays out the batch block at `mem` whose `count` segments are already
+// written after its header: prefix sums of tiles of `tile` outputs, the
+// tile -> segment map (at most `max_tiles` entries) and the header.  Sets
+// *ntiles to the tile count and returns the bytes in use.
+size_t layout_batch_block(uint8_t* mem, uint32_t count, uint64_t tile, uint64_t max_tiles, uint64_t* ntiles);
+
 class BatchSlot {
  public:
   BatchSlot() = default;
@@ -79,7 +87,6 @@ class BatchSlot {
   uint8_t* dmem_ = nullptr;
   uint8_t* hmem_ = nullptr;
   size_t used_bytes_ = 0;  // staged prefix to copy
-  BatchHeader* h_hdr_ = nullptr;
   BatchSegment* h_segs_ = nullptr;
   hipEvent_t staged_ = nullptr, run_done_ = nullptr;
   bool copy_pending_ = false, run_pending_ = false, dirty_ = false;
