@@ -238,12 +238,34 @@ def bench_batched(args):
     for _ in range(2):
         b.run(s)
     km = time_launches(lambda: b.run(s), s, max(5, args.steps))
-    buf.free()
     gbs = total * 8 / (km * 1e-3) / 1e9
     log(f"batched: {total / (km * 1e-3) / 1e9:.1f} G samples/s, {gbs:.0f} GB/s, cfg {cfg}")
-    return {"streams": len(lengths), "outputs": total, "ms_per_launch": km, "gsps": total / (km * 1e-3) / 1e9,
-            "gbs": gbs, "frac_of_peak": gbs / PEAK_GBS, "config": cfg,
-            "launch": "direct launch, " + ("one tile per wave" if cfg["blocks_per_cu"] == 0 else "persistent grid")}
+    out = {"streams": len(lengths), "outputs": total, "ms_per_launch": km, "gsps": total / (km * 1e-3) / 1e9,
+           "gbs": gbs, "frac_of_peak": gbs / PEAK_GBS, "config": cfg,
+           "launch": "direct launch, " + ("one tile per wave" if cfg["blocks_per_cu"] == 0 else "persistent grid")}
+    # the configuration's named form: one replay of a captured HIP graph per
+    # run, with a persistent grid (best of a few shapes) and with one tile per wave
+    b.use_graph(True)
+    for label, shapes in (("persistent_graph", [(256, 4, 4096), (256, 8, 4096), (512, 2, 4096), (256, 4, 8192),
+                                                  (1024, 1, 4096), (256, 8, 2048)]),
+                          ("tiles_graph", [(1024, 0, 2048)])):
+        best = None
+        for shape in shapes:
+            b.set_config(*shape)
+            for _ in range(2):
+                b.run(s)
+            t = time_launches(lambda: b.run(s), s, max(5, args.steps))
+            if best is None or t < best[1]:
+                best = (shape, t)
+        g = total * 8 / (best[1] * 1e-3) / 1e9
+        out[label] = {"ms_per_launch": round(best[1], 4), "gsps": round(total / (best[1] * 1e-3) / 1e9, 2),
+                      "gbs": round(g, 1), "frac_of_peak": round(g / PEAK_GBS, 4),
+                      "config": dict(zip(("block_threads", "blocks_per_cu", "tile_outputs"), best[0])),
+                      "shapes_tried": len(shapes)}
+        log(f"batched {label}: {best[1]:.3f} ms, {g:.0f} GB/s, shape {best[0]}")
+    b.use_graph(False)
+    buf.free()
+    return out
 
 
 def bench_jumps(args):

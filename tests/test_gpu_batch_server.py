@@ -222,31 +222,40 @@ def test_batch_errors(gpu):
         b.set_config(256, 0, 2**64 - 256)
 
 
-@pytest.mark.parametrize("shape", [None, (1024, 0, 4096)], ids=["default", "tile-per-wave"])
-def test_c3_batched_full_size(gpu, oracle_mod, shape):
-    """4096 streams of log-uniform length in [2^10, 2^22), float64, one launch;
-    every stream checked by its position-weighted checksum."""
+@pytest.mark.parametrize("shape,graph", [(None, False), ((1024, 0, 4096), False), ((256, 4, 4096), True),
+                                         ((1024, 0, 2048), True)],
+                         ids=["default", "tile-per-wave", "persistent-graph", "tile-per-wave-graph"])
+def test_c3_batched_full_size(gpu, oracle_mod, shape, graph):
+    """4096 streams of log-uniform length in [2^10, 2^22), float64, one launch
+    (a plain launch, or a replay of the captured graph, replayed twice with the
+    output cleared in between); every stream checked by its position-weighted
+    checksum."""
     from mrg8_amd import workloads
     from mrg8_amd.hip import DeviceBuffer
 
     states, lengths = workloads.batched()
     total = int(lengths.sum())
-    say("c3", total)
+    say("c3", total, shape, "graph" if graph else "direct")
     buf = DeviceBuffer(total * 8)
     starts = np.concatenate([[0], np.cumsum(lengths)[:-1]]).astype(np.uint64)
     b = gpu.Batch(max_streams=len(lengths))
     if shape:
         b.set_config(*shape)
+    b.use_graph(graph)
     b.set([(states[i], 0, int(lengths[i]), buf.ptr + 8 * int(starts[i]), 0) for i in range(len(lengths))])
-    b.run()
-    gpu.hip.synchronize()
-    host = buf.copy_to_host(np.float64, total)
-    buf.free()
     want = oracle_mod.stream_wsums(states, np.zeros(len(lengths), dtype=np.uint64), lengths,
                                    np.zeros(len(lengths), dtype=np.int32))
-    bad = [i for i in range(len(lengths))
-           if oracle_mod.wsum(host[int(starts[i]):int(starts[i] + lengths[i])]) != int(want[i])]
-    assert not bad, bad[:10]
+    for replay in range(2 if graph else 1):
+        if replay:
+            buf.fill_byte(0)  # the second replay must write every output again
+        b.run()
+        gpu.hip.synchronize()
+        host = buf.copy_to_host(np.float64, total)
+        bad = [i for i in range(len(lengths))
+               if oracle_mod.wsum(host[int(starts[i]):int(starts[i] + lengths[i])]) != int(want[i])]
+        assert not bad, (replay, bad[:10])
+        del host
+    buf.free()
 
 
 @pytest.mark.parametrize("shape", [(256, 8, 256), (1024, 0, 256)], ids=["persistent", "tile-per-wave"])
