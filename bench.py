@@ -285,10 +285,23 @@ def bench_jumps(args):
     km = time_launches(lambda: b.run(s), s, max(5, args.steps))
     buf.free()
     pc = float(np.mean([bin(int(x)).count("1") for x in steps]))
-    log(f"jumps: {km:.3f} ms per launch, {n / (km * 1e-3) / 1e6:.1f} M streams/s")
+    digits = float(np.mean([sum(1 for k in range(8) if (int(x) >> (8 * k)) & 0xFF) for x in steps]))
+    # multiply-adds: the reference's jump (one 8x8 matrix-vector product, 64
+    # MACs, per set bit, knl/mrg8.h:212-238) plus 8 per output; and what the
+    # kernel executes (one product per non-zero byte of the step)
+    macs_ref = n * (pc * 64 + 256 * 8)
+    macs_run = n * (digits * 64 + 256 * 8)
+    hbm = n * (256 * 4 + 64)  # outputs + one 64-byte segment record per stream
+    log(f"jumps: {km:.3f} ms per launch, {n / (km * 1e-3) / 1e6:.1f} M streams/s, "
+        f"{macs_run / (km * 1e-3) / 1e12:.2f} T MACs/s")
     return {"streams": n, "outputs_per_stream": 256, "ms_per_launch": km, "streams_per_s": n / (km * 1e-3),
-            "gsps": n * 256 / (km * 1e-3) / 1e9, "mean_popcount": pc,
-            "jump_macs_per_s": n * pc * 64 / (km * 1e-3)}
+            "gsps": n * 256 / (km * 1e-3) / 1e9, "mean_popcount": pc, "mean_nonzero_bytes": digits,
+            "jump_macs_per_s": n * pc * 64 / (km * 1e-3),
+            "valu": {"bound": "valu", "unit": "lane MACs/s (32x32->64)",
+                     "executed_macs_per_launch": macs_run, "reference_algorithm_macs_per_launch": macs_ref,
+                     "achieved": round(macs_run / (km * 1e-3), 1)},
+            "hbm": {"bytes_per_launch": hbm, "gbs": round(hbm / (km * 1e-3) / 1e9, 1),
+                    "frac": round(hbm / (km * 1e-3) / 1e9 / PEAK_GBS, 4)}}
 
 
 def bench_advance(args, n=1 << 20):
@@ -542,6 +555,18 @@ def main():
         line["rocrand_f64_2e28"] = {k: {kk: round(vv, 4) for kk, vv in v.items()} if "error" not in v else v
                                     for k, v in rates.items()}
         log("rocrand: " + ", ".join(f"{k} {v.get('steady_gsps', 0):.1f}" for k, v in rates.items()))
+    if not args.quick:
+        # the VALU ceiling of the jump-heavy batch, measured last: a
+        # multiply-add-only burst lowers the clock for the launches after it
+        from mrg8_amd import hip, probe
+
+        s = hip.Stream()
+        peak = probe.mad64_rate(s)
+        s.destroy()
+        v = line["jumps"]["valu"]
+        v.update(peak_measured=round(peak, 1), frac=round(v["achieved"] / peak, 4),
+                 peak_kernel="mrg8_probe_mad64: 8 independent v_mad_i64_i32 chains per lane, run after every GPU leg")
+        log(f"jumps VALU: {v['achieved'] / 1e12:.2f} of {peak / 1e12:.1f} T MACs/s measured peak")
     if not args.no_cpu:
         line.update(cpu_baselines(args))
     print(json.dumps(line), flush=True)

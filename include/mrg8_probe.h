@@ -19,6 +19,13 @@ extern "C" {
  * non-temporal wave stores; d 1 KiB-aligned) and no arithmetic. */
 int mrg8_probe_write(void* d, uint64_t n, int mode, uint64_t tile, void* stream);
 
+/* `blocks` blocks of 256 threads, each thread running eight independent
+ * chains of `iters` 32x32->64 signed multiply-adds (v_mad_i64_i32) and
+ * writing one word to d_out[thread] (d_out: blocks * 256 words): the
+ * lane-multiply-adds per second the VALU sustains, blocks * 256 * 8 * iters
+ * divided by the kernel time. */
+int mrg8_probe_mad64(uint64_t* d_out, uint32_t blocks, int iters, void* stream);
+
 /* One wave spins ~20 us and writes d_out[0] = shader-clock ticks
  * (s_memtime) and d_out[1] = 100 MHz reference ticks (s_memrealtime) over
  * the same interval: clock in MHz = 100 * d_out[0] / d_out[1]. */

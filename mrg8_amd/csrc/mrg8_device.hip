@@ -105,6 +105,26 @@ __device__ __forceinline__ void dot_residues(const Coef (&k)[PL], const int32_t 
   }
 }
 
+// The value of v in each of the eight lanes of this lane's 8-lane group,
+// w[j] from lane (lane & ~7) | j: ds_swizzle in bit mode (and-mask 0x18 keeps
+// the group bits within the 32-lane half, or-mask j picks the lane), so no
+// lane-address registers are needed.
+template <int J>
+__device__ __forceinline__ uint32_t group_lane(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, 0x18 | (J << 5));
+}
+
+__device__ __forceinline__ void gather_group(uint32_t v, uint32_t w[8]) {
+  w[0] = group_lane<0>(v);
+  w[1] = group_lane<1>(v);
+  w[2] = group_lane<2>(v);
+  w[3] = group_lane<3>(v);
+  w[4] = group_lane<4>(v);
+  w[5] = group_lane<5>(v);
+  w[6] = group_lane<6>(v);
+  w[7] = group_lane<7>(v);
+}
+
 __device__ __forceinline__ void centre(const uint32_t s[8], int32_t e[8]) {
 #pragma unroll
   for (int j = 0; j < 8; ++j) e[j] = (int32_t)(s[j] - kCenter);
@@ -114,7 +134,7 @@ __device__ __forceinline__ void centre(const uint32_t s[8], int32_t e[8]) {
 // (replicated in the group's lanes, s[] per lane) and its own 64-bit offset;
 // for the group's lowest non-zero digit d (kJumpBits bits) at position k,
 // lane r of the group computes row r of A^(d 2^(kJumpBits k)), and the
-// group's eight new words are gathered back with ds_bpermute.  Groups with no
+// group's eight new words are gathered back with ds_swizzle.  Groups with no
 // digits left idle until every group is done; a 64-bit jump takes at most
 // kJumpPositions steps.
 __device__ __forceinline__ const CoefRecord* digit_row(const CoefRecord* __restrict__ jrec, uint64_t off, int row) {
@@ -128,8 +148,7 @@ __device__ __forceinline__ uint64_t drop_digit(uint64_t off) {
 }
 
 __device__ __forceinline__ void jump_groups(uint32_t s[8], uint64_t off, const CoefRecord* __restrict__ jrec) {
-  const int lane = threadIdx.x & 63;
-  const int grp = lane >> 3, row = lane & 7;
+  const int row = threadIdx.x & 7;
   Coef k = {};
   if (off) k = load_coef(digit_row(jrec, off, row));
   while (__ballot(off != 0)) {
@@ -143,11 +162,10 @@ __device__ __forceinline__ void jump_groups(uint32_t s[8], uint64_t off, const C
 #pragma unroll
     for (int j = 0; j < 8; ++j) nz |= (s[j] != 0u ? 1u : 0u) << j;
     const uint32_t y = (k.support & nz) ? dot_residue(k, e) + 1u : 0u;
+    uint32_t w[8];
+    gather_group(y, w);
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      const uint32_t w = (uint32_t)__builtin_amdgcn_ds_bpermute((grp * 8 + j) * 4, (int)y);
-      s[j] = active ? w : s[j];
-    }
+    for (int j = 0; j < 8; ++j) s[j] = active ? w[j] : s[j];
     off = rest;
     k = next;
   }
@@ -181,7 +199,7 @@ __device__ __forceinline__ const CoefRecord* digit_rec(const CoefRecord* __restr
 // the group's matrix (record c) times the group's state, and the group's
 // eight new words are gathered back.
 template <bool ZERO_WORDS = true>
-__device__ __forceinline__ void jump_step(uint32_t s[8], const Coef& c, int grp) {
+__device__ __forceinline__ void jump_step(uint32_t s[8], const Coef& c) {
   int32_t e[8];
   centre(s, e);
   uint32_t y = dot_residue(c, e) + 1u;
@@ -196,21 +214,19 @@ __device__ __forceinline__ void jump_step(uint32_t s[8], const Coef& c, int grp)
     for (int j = 0; j < 8; ++j) nz |= (s[j] != 0u ? 1u : 0u) << j;
     if (!(c.support & nz)) y = 0u;
   }
-#pragma unroll
-  for (int j = 0; j < 8; ++j) s[j] = (uint32_t)__builtin_amdgcn_ds_bpermute((grp * 8 + j) * 4, (int)y);
+  gather_group(y, s);
 }
 
 __device__ __forceinline__ void jump_groups_pipelined(uint32_t s[8], uint64_t off,
                                                       const CoefRecord* __restrict__ jrec) {
-  const int lane = threadIdx.x & 63;
-  const int grp = lane >> 3, row = lane & 7;
+  const int row = threadIdx.x & 7;
   if (!__ballot(off != 0)) return;
   Coef c = load_coef11(digit_rec(jrec, off, 0, row));
   for (int k = 0;; ++k) {
     // wave-uniform: no group has a digit above position k
     const bool last = k + 1 == kJumpPositions || !__ballot((off >> (kJumpBits * (k + 1))) != 0);
     const Coef next = load_coef11(digit_rec(jrec, last ? 0 : off, last ? 0 : k + 1, row));
-    if (__ballot(digit_at(off, k) != 0)) jump_step(s, c, grp);
+    if (__ballot(digit_at(off, k) != 0)) jump_step(s, c);
     if (last) break;
     c = next;
   }
@@ -609,15 +625,17 @@ __device__ __forceinline__ const CoefRecord* wide_rec(const AdvanceArgs& a, int 
 // jump_step for states with no zero word, on centred words e = s - 2^30:
 // each lane centres its one new word before the gather instead of every lane
 // centring eight after it.
-__device__ __forceinline__ void jump_step_centred(int32_t e[8], const Coef& c, int grp) {
-  const int32_t ey = (int32_t)(dot_residue(c, e) + 1u - kCenter);
+__device__ __forceinline__ void jump_step_centred(int32_t e[8], const Coef& c) {
+  const uint32_t ey = dot_residue(c, e) + 1u - kCenter;
+  uint32_t w[8];
+  gather_group(ey, w);
 #pragma unroll
-  for (int j = 0; j < 8; ++j) e[j] = __builtin_amdgcn_ds_bpermute((grp * 8 + j) * 4, ey);
+  for (int j = 0; j < 8; ++j) e[j] = (int32_t)w[j];
 }
 
 template <bool ZERO_WORDS>
 __device__ __forceinline__ void advance_digits(uint32_t s[8], const uint64_t (&off)[kAdvanceWords],
-                                               const AdvanceArgs& a, int grp, int row) {
+                                               const AdvanceArgs& a, int row) {
   const int npos = 8 * (int)a.words;
   int32_t e[8];
   if (!ZERO_WORDS) centre(s, e);
@@ -638,9 +656,9 @@ __device__ __forceinline__ void advance_digits(uint32_t s[8], const uint64_t (&o
       const Coef next = load_coef11(wide_rec(a, last ? 0 : k + 1, dn, row));
       if (__ballot(d != 0u)) {
         if (ZERO_WORDS)
-          jump_step<true>(s, c, grp);
+          jump_step<true>(s, c);
         else
-          jump_step_centred(e, c, grp);
+          jump_step_centred(e, c);
       }
       c = next;
       d = dn;
@@ -653,8 +671,7 @@ __device__ __forceinline__ void advance_digits(uint32_t s[8], const uint64_t (&o
 }
 
 __global__ __launch_bounds__(256) void k_advance(AdvanceArgs a) {
-  const int lane = threadIdx.x & 63;
-  const int grp = lane >> 3, row = lane & 7;
+  const int row = threadIdx.x & 7;
   const uint64_t item = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 3;
   const bool live = item < a.n;
   uint32_t s[8];
@@ -670,9 +687,9 @@ __global__ __launch_bounds__(256) void k_advance(AdvanceArgs a) {
   // a state with no zero word never gets one (no row of a power of A is
   // zero), so the zero-product test is needed only by waves that start with one
   if (__ballot(smin == 0u))
-    advance_digits<true>(s, off, a, grp, row);
+    advance_digits<true>(s, off, a, row);
   else
-    advance_digits<false>(s, off, a, grp, row);
+    advance_digits<false>(s, off, a, row);
   if (live && row == 0) {
 #pragma unroll
     for (int j = 0; j < 8; ++j) a.out[8 * item + j] = s[j];

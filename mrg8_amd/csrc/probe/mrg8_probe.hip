@@ -6,6 +6,9 @@
 //    stores); mode 1 writes with the store pattern of the generator's fill
 //    kernel and no arithmetic (each wave one run of `tile` doubles, 16 bytes
 //    per lane and 1 KiB per wave store, non-temporal).
+//  * 64-bit multiply-add peak: every lane runs eight independent chains of
+//    v_mad_i64_i32 (the generator's multiply-add), the VALU-side ceiling of
+//    its integer work.
 //  * Shader-clock probe: one wave counts s_memtime (shader clock) ticks
 //    against the constant 100 MHz s_memrealtime over about 20 us, so a
 //    caller can see the clock the power manager has set between launches.
@@ -42,6 +45,26 @@ __global__ __launch_bounds__(1024) void k_write_tiles(double* __restrict__ d, ui
   for (uint64_t k = i + lane; k < end; k += 64) __builtin_nontemporal_store((double)k, d + k);
 }
 
+__global__ __launch_bounds__(256) void k_mad64(uint64_t* __restrict__ out, int iters) {
+  const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+  uint64_t acc[8];
+  int32_t u[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    acc[j] = t + j;
+    u[j] = (int32_t)(t ^ (j * 0x9e3779b9u));
+  }
+  for (int i = 0; i < iters; ++i) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j)  // acc = lo32(acc) * u + acc: each chain depends on itself only
+      acc[j] = (uint64_t)((int64_t)(int32_t)(uint32_t)acc[j] * (int64_t)u[j]) + acc[j];
+  }
+  uint64_t sum = 0;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) sum += acc[j];
+  out[t] = sum;  // keeps the chains live
+}
+
 __global__ void k_sclk(uint64_t* __restrict__ out) {
   const uint64_t r0 = __builtin_amdgcn_s_memrealtime();
   const uint64_t t0 = __builtin_amdgcn_s_memtime();
@@ -76,6 +99,12 @@ extern "C" int mrg8_probe_write(void* d, uint64_t n, int mode, uint64_t tile, vo
   } else {
     return -1;
   }
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+extern "C" int mrg8_probe_mad64(uint64_t* d_out, uint32_t blocks, int iters, void* stream) {
+  if (!d_out || blocks == 0 || iters <= 0) return -1;
+  hipLaunchKernelGGL(k_mad64, dim3(blocks), dim3(256), 0, static_cast<hipStream_t>(stream), d_out, iters);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 

@@ -22,6 +22,7 @@ def lib() -> ctypes.CDLL:
         L.mrg8_probe_write.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_uint64,
                                        ctypes.c_void_p]
         L.mrg8_probe_sclk.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        L.mrg8_probe_mad64.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int, ctypes.c_void_p]
         _lib = L
     return _lib
 
@@ -36,6 +37,30 @@ def write(dptr: int, n: int, mode: int = 1, tile: int = 0, stream=None) -> None:
     rc = lib().mrg8_probe_write(ctypes.c_void_p(int(dptr)), n, mode, tile, _s(stream))
     if rc != 0:
         raise RuntimeError(f"mrg8_probe_write failed ({rc})")
+
+
+def mad64_rate(stream, blocks: int = 4096, iters: int = 2048, reps: int = 5) -> float:
+    """Sustained 32x32->64 multiply-adds per second over the whole GPU
+    (eight independent v_mad_i64_i32 chains per lane)."""
+    from . import hip
+
+    out = hip.DeviceBuffer(blocks * 256 * 8)
+    try:
+        launch = lambda: lib().mrg8_probe_mad64(ctypes.c_void_p(out.ptr), blocks, iters, _s(stream))  # noqa: E731
+        if launch() != 0:
+            raise RuntimeError("mrg8_probe_mad64 failed")
+        e0, e1 = hip.Event(), hip.Event()
+        e0.record(stream)
+        for _ in range(reps):
+            launch()
+        e1.record(stream)
+        e1.synchronize()
+        ms = e0.elapsed_ms(e1) / reps
+        e0.destroy()
+        e1.destroy()
+    finally:
+        out.free()
+    return blocks * 256 * 8 * iters / (ms * 1e-3)
 
 
 class ClockProbe:

@@ -49,13 +49,15 @@ def test_probe_library_exports_its_header():
     from mrg8_amd import probe
 
     names = header_functions(only="mrg8_probe.h")
-    assert names == {"mrg8_probe_write", "mrg8_probe_sclk"}
+    assert names == {"mrg8_probe_write", "mrg8_probe_sclk", "mrg8_probe_mad64"}
     assert all(hasattr(probe.lib(), n) for n in names)
     # argument checks come before any device work
     assert probe.lib().mrg8_probe_write(None, 8, 0, 0, None) == -1
     assert probe.lib().mrg8_probe_write(ctypes.c_void_p(1024), 8, 1, 100, None) == -1  # tile not a multiple of 128
     assert probe.lib().mrg8_probe_write(ctypes.c_void_p(1024), 8, 7, 0, None) == -1
     assert probe.lib().mrg8_probe_sclk(None, None) == -1
+    assert probe.lib().mrg8_probe_mad64(None, 4, 4, None) == -1
+    assert probe.lib().mrg8_probe_mad64(ctypes.c_void_p(1024), 0, 4, None) == -1
 
 
 def test_library_is_gfx950_only():
