@@ -15,6 +15,13 @@ last `--reps` dispatches of each kind):
   5. jump-heavy batched launch: 65536 x (63-bit jump + 256 uint32) (--batched)
   6. bulk advance of 2^20 device-resident states by random 200-bit offsets,
      then by random 64-bit offsets                                 (--advance)
+
+With --clock (no profiler needed) it instead prints, per launch, the kernel
+time (HIP events) and the shader clock sampled right after it, for a
+sequence started after 2 s idle: six 2^28 float64 fills, then twelve rounds
+of { fill, write probe with the fill's store pattern, hipMemsetAsync, 2^28
+uint32 fill } over the same 2 GiB -- which kernels slow down while the power
+manager holds the clock down.
 """
 from __future__ import annotations
 
@@ -33,7 +40,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--batched", action="store_true")
     ap.add_argument("--advance", action="store_true")
+    ap.add_argument("--clock", action="store_true")
     args = ap.parse_args()
+    if args.clock:
+        clock_interleave()
+        return
 
     import mrg8_amd
     from mrg8_amd import hip, workloads
@@ -98,6 +109,40 @@ def main():
         d_in.free()
         d_out.free()
     print("profile workload done", flush=True)
+
+
+def clock_interleave():
+    import time
+
+    import mrg8_amd
+    from mrg8_amd import hip, probe
+
+    mrg8_amd.capi.call("mrg8_init_device", 0)
+    n = 1 << 28
+    s = hip.Stream()
+    buf = hip.DeviceBuffer(n * 8)
+    g = mrg8_amd.mrg8_cuda(13579)
+    seq = [("fill f64", lambda: g.generate(buf, n, mrg8_amd.F64, stream=s))] * 6
+    seq += [("fill f64", lambda: g.generate(buf, n, mrg8_amd.F64, stream=s)),
+            ("write probe", lambda: probe.write(buf.ptr, n, 1, 0, s)),
+            ("memset", lambda: hip.memset_async(buf.ptr, 0, n * 8, s)),
+            ("fill u32", lambda: g.generate(buf, n, mrg8_amd.U32, stream=s))] * 12
+    clk = probe.ClockProbe(len(seq))
+    ev = [(hip.Event(), hip.Event()) for _ in seq]
+    s.synchronize()
+    time.sleep(2.0)
+    for (name, fn), (e0, e1) in zip(seq, ev):
+        e0.record(s)
+        fn()
+        e1.record(s)
+        clk.sample(s)
+    s.synchronize()
+    mhz = clk.mhz()
+    print("launch  kernel        us    MHz")
+    for i, ((name, _), (e0, e1)) in enumerate(zip(seq, ev)):
+        print(f"{i:6d}  {name:11s} {e0.elapsed_ms(e1) * 1e3:7.1f} {mhz[i]:6.0f}", flush=True)
+    clk.free()
+    buf.free()
 
 
 if __name__ == "__main__":
