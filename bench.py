@@ -7,15 +7,23 @@ reference's `rand` conversion) of a single stream seeded with 13579, split
 across all waves by jump-ahead -- the single-stream workload of the
 benchmark configurations.  Printed value: G samples/s over the timed steps.
 
-The same JSON line carries, measured in the same run:
+The same JSON line carries, measured in the same run (in this order):
+  write_ceiling the project's write probe (d[i] = i, the counterpart of the
+                reference's cuda/stream.cu) over the same 2 GiB with the fill's
+                store pattern and in the reference's form, and hipMemsetAsync
   roofline      algorithmic bytes per launch / mean kernel time (HIP events on
-                the launch stream) against 8 TB/s; HBM traffic from the
+                the launch stream, one between every two launches; every
+                launch's duration listed) against 8 TB/s; HBM traffic from the
                 committed rocprofv3 counter summary when present
   u32           the same workload in raw uint32 words
   small         2^24 float64 outputs (the reference's full-compare size)
+  clock         the float64 fill from idle with the shader clock sampled after
+                each launch (the power manager's clamp), then the uint32 fill
   batched       4096 streams, log-uniform lengths in [2^10, 2^22), float64, one
-                launch with the autotuned shape
-  jumps         65536 streams jumped by uniform 63-bit steps, 256 uint32 each
+                launch with the autotuned shape; also as one replay of a
+                captured HIP graph (persistent grid, one tile per wave)
+  jumps         65536 streams jumped by uniform 63-bit steps, 256 uint32 each,
+                with the multiply-add rate against the measured VALU peak
   advance       2^20 device-resident states jumped by 200-bit (and 64-bit)
                 offsets (mrg8_advance_states)
   served        1e5 Poisson requests on 16384 resident streams, sizes
@@ -163,9 +171,34 @@ def bench_single(args):
     out["small"] = {"kernel_ms": km, "gsps": small / (km * 1e-3) / 1e9,
                     "gbs": small * 8 / (km * 1e-3) / 1e9}
     out["clock"] = clock_series(g, buf, s)
+    # the word fill again right after that series, with a clock sample after
+    # each launch: the same kernel once the clock has recovered (its timed
+    # window above follows the float64 leg, inside the clamp)
+    out["u32_after_clock_series"] = clock_tail(g, buf, s, mrg8_amd.U32, 30)
     buf.free()
     s.destroy()
     return out
+
+
+def clock_tail(g, buf, s, fmt, launches):
+    from mrg8_amd import hip, probe
+
+    clk = probe.ClockProbe(launches)
+    ev = [hip.Event() for _ in range(2 * launches)]
+    for i in range(launches):
+        ev[2 * i].record(s)
+        g.generate(buf, N_SINGLE, fmt, stream=s)
+        ev[2 * i + 1].record(s)
+        clk.sample(s)
+    s.synchronize()
+    us = [ev[2 * i].elapsed_ms(ev[2 * i + 1]) * 1e3 for i in range(launches)]
+    mhz = clk.mhz()
+    clk.free()
+    for e in ev:
+        e.destroy()
+    k = us[-15:]
+    return {"launches": launches, "last15_kernel_ms": round(sum(k) / len(k) * 1e-3, 4),
+            "last15_sclk_mhz": round(sum(mhz[-15:]) / 15, 1), "min_sclk_mhz": min(mhz)}
 
 
 def clock_series(g, buf, s, idle_s=2.0, launches=60):
@@ -349,9 +382,16 @@ def bench_served(args, capacity_gsps):
     # warm-up trace (separate streams' positions advance; states stay resident)
     srv.replay(arrivals[:200] / 10, ids[:200], sizes[:200], fmt=mrg8_amd.F64)
     c0 = srv.counters()
-    stats = srv.replay(arrivals, ids, sizes, fmt=mrg8_amd.F64)
+    stats, lat = srv.replay(arrivals, ids, sizes, fmt=mrg8_amd.F64, want_latencies=True)
     c1 = srv.counters()
     srv.close()
+    # where in the trace the slow requests arrived: p99 per 100 ms of arrivals
+    windows = []
+    for w0 in np.arange(0.0, float(arrivals[-1]) + 1e-9, 0.1):
+        m = (arrivals >= w0) & (arrivals < w0 + 0.1)
+        if m.any():
+            windows.append(round(float(np.quantile(lat[m], 0.99)) * 1e3, 3))
+    worst = int(np.argmax(lat))
     steps = max(1, c1["steps"] - c0["steps"])
     out = {"requests": n_req, "streams": len(states), "offered_load": args.served_load,
            "arrival_rate_per_s": rate, "outputs": int(stats["outputs"]), "wall_s": stats["wall_s"],
@@ -361,7 +401,12 @@ def bench_served(args, capacity_gsps):
            "steps": steps, "mean_step_outputs": (c1["step_outputs"] - c0["step_outputs"]) / steps,
            "mean_step_segments": (c1["step_segments"] - c0["step_segments"]) / steps,
            "host_us_per_step": (c1["host_launch_s"] - c0["host_launch_s"]) / steps * 1e6,
-           "max_admission_wait_ms": c1["max_queue_s"] * 1e3}
+           "max_admission_wait_ms": c1["max_queue_s"] * 1e3,
+           "p99_ms_per_100ms_of_arrivals": windows,
+           "worst": {"arrival_ms": round(float(arrivals[worst]) * 1e3, 3), "latency_ms": round(float(lat[worst]) * 1e3, 3),
+                     "outputs": int(sizes[worst])},
+           "max_host_gap_ms": round(stats["max_host_gap_s"] * 1e3, 3),
+           "max_host_gap_at_ms": round(stats["max_host_gap_at_s"] * 1e3, 3)}
     log(f"served: {out['gsps']:.1f} G samples/s, p50 {out['p50_ms']:.3f} ms, p99 {out['p99_ms']:.3f} ms")
     return out
 
@@ -526,7 +571,10 @@ def main():
                 "kernel_ms": round(single["u32"]["kernel_ms"], 4),
                 "gbs": round(single["u32"]["bytes"] / (single["u32"]["kernel_ms"] * 1e-3) / 1e9, 1),
                 "frac": round(single["u32"]["bytes"] / (single["u32"]["kernel_ms"] * 1e-3) / 1e9 / PEAK_GBS, 4),
-                "per_launch_ms": [round(x, 4) for x in single["u32"]["per_launch_ms"]]},
+                "per_launch_ms": [round(x, 4) for x in single["u32"]["per_launch_ms"]],
+                "after_clock_series": dict(single["u32_after_clock_series"], frac=round(
+                    single["u32"]["bytes"] / single["u32_after_clock_series"]["last15_kernel_ms"] / 1e6 / PEAK_GBS,
+                    4))},
         "small_2e24_f64": {k: round(v, 4) for k, v in single["small"].items()},
         "write_ceiling": {"kernel": "mrg8_probe_write: d[i] = i over the same 2 GiB with the fill's 1 KiB "
                                     "non-temporal wave stores (the counterpart of cuda/stream.cu), measured "

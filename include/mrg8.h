@@ -363,6 +363,8 @@ typedef struct mrg8_trace_stats {
   uint64_t outputs;
   double wall_s;
   double p50_s, p90_s, p99_s, max_s, mean_s;
+  double max_host_gap_s;   /* longest time between two passes of the replay loop */
+  double max_host_gap_at_s; /* when that gap ended, seconds from the start of the replay */
 } mrg8_trace_stats;
 
 int mrg8_server_replay(mrg8_server_t s, const mrg8_trace_entry* trace, uint64_t n, double* latency_s,

@@ -59,7 +59,8 @@ class TraceEntry(ctypes.Structure):
 class TraceStats(ctypes.Structure):
     _fields_ = [("completed", ctypes.c_uint64), ("outputs", ctypes.c_uint64), ("wall_s", ctypes.c_double),
                 ("p50_s", ctypes.c_double), ("p90_s", ctypes.c_double), ("p99_s", ctypes.c_double),
-                ("max_s", ctypes.c_double), ("mean_s", ctypes.c_double)]
+                ("max_s", ctypes.c_double), ("mean_s", ctypes.c_double), ("max_host_gap_s", ctypes.c_double),
+                ("max_host_gap_at_s", ctypes.c_double)]
 
 
 u32p = ctypes.POINTER(ctypes.c_uint32)

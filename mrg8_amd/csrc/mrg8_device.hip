@@ -696,6 +696,19 @@ __global__ __launch_bounds__(256) void k_advance(AdvanceArgs a) {
   }
 }
 
+// Staging copy of a batch block: the host writes the block into coherent
+// pinned memory and this kernel moves it to device memory (16 bytes per
+// thread, then 4-byte words for the tail).  On MI355X the runtime's
+// host-to-device copy of these few kilobytes occasionally blocked the
+// calling thread for 6-7 ms, where a kernel launch returns in microseconds.
+__global__ __launch_bounds__(256) void k_stage(const u4_t* __restrict__ src, u4_t* __restrict__ dst, uint64_t n16,
+                                                uint64_t tail_words) {
+  const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n16) dst[i] = src[i];
+  if (i < tail_words)
+    reinterpret_cast<uint32_t*>(dst + n16)[i] = reinterpret_cast<const uint32_t*>(src + n16)[i];
+}
+
 template <int FMT, int PL>
 hipError_t fill_one(const FillArgs& a, int block_threads, hipStream_t stream) {
   if (a.n == 0) return hipSuccess;
@@ -735,6 +748,18 @@ hipError_t launch_advance(const AdvanceArgs& a, hipStream_t stream) {
   if (a.n > (0xffffffffull >> 3) || a.words < 1 || a.words > (uint32_t)kAdvanceWords) return hipErrorInvalidValue;
   const uint64_t blocks = (threads + 255) / 256;
   hipLaunchKernelGGL(k_advance, dim3((uint32_t)blocks), dim3(256), 0, stream, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_stage(const void* src, void* dst, size_t bytes, hipStream_t stream) {
+  if (bytes == 0) return hipSuccess;
+  if (bytes % 4 || (reinterpret_cast<uintptr_t>(src) | reinterpret_cast<uintptr_t>(dst)) % 16)
+    return hipErrorInvalidValue;
+  const uint64_t n16 = bytes / 16, tail = (bytes % 16) / 4;
+  const uint64_t blocks = (n16 + 255) / 256 + (n16 == 0 ? 1 : 0);
+  if (blocks > 0xffffffffull) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_stage, dim3((uint32_t)blocks), dim3(256), 0, stream, static_cast<const u4_t*>(src),
+                     static_cast<u4_t*>(dst), n16, tail);
   return hipGetLastError();
 }
 

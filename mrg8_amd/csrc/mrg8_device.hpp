@@ -161,5 +161,8 @@ hipError_t launch_batch(const BatchArgs& a, int grid, int block_threads, hipStre
 hipError_t launch_batch_tiles(const BatchArgs& a, uint64_t ntiles, int block_threads, hipStream_t stream);
 // Eight lanes per state; a.n * 8 must stay below 2^32.
 hipError_t launch_advance(const AdvanceArgs& a, hipStream_t stream);
+// Copies `bytes` (a multiple of 4) from coherent pinned host memory `src` to
+// device memory `dst`, both 16-byte aligned, with a kernel on `stream`.
+hipError_t launch_stage(const void* src, void* dst, size_t bytes, hipStream_t stream);
 
 }  // namespace mrg8

@@ -535,8 +535,14 @@ extern "C" int mrg8_server_replay(mrg8_server_t s, const mrg8_trace_entry* trace
   std::vector<mrg8_completion> done(1024);
   uint64_t next = 0, finished = 0, outputs = 0;
   const double t0 = wall_now();
+  double prev = 0.0, max_gap = 0.0, max_gap_at = 0.0;
   while (finished < n) {
     const double now = wall_now() - t0;
+    if (now - prev > max_gap) {
+      max_gap = now - prev;
+      max_gap_at = now;
+    }
+    prev = now;
     while (next < n && trace[next].arrival_s <= now) {
       uint64_t id = 0;
       int rc = mrg8_server_submit(s, trace[next].stream_id, trace[next].count, trace[next].format, &id);
@@ -576,6 +582,8 @@ extern "C" int mrg8_server_replay(mrg8_server_t s, const mrg8_trace_entry* trace
     stats->p99_s = pct(0.99);
     stats->max_s = sorted.empty() ? 0.0 : sorted.back();
     stats->mean_s = n ? sum / (double)n : 0.0;
+    stats->max_host_gap_s = max_gap;
+    stats->max_host_gap_at_s = max_gap_at;
   }
   return MRG8_OK;
 }

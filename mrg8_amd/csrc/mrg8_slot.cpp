@@ -24,7 +24,9 @@ int BatchSlot::init(DeviceContext* ctx, uint32_t cap, uint64_t max_tiles, const 
   const size_t total = align_up(tile_seg_offset(cap) + sizeof(uint32_t) * (size_t)max_tiles, 256);
   DeviceGuard dg(ctx->device);
   hipError_t e = hipMalloc((void**)&dmem_, total);
-  if (e == hipSuccess) e = hipHostMalloc((void**)&hmem_, total, hipHostMallocDefault);
+  // coherent: the staging kernel reads it over the bus every run, never a
+  // copy the GPU cached from an earlier run of this slot
+  if (e == hipSuccess) e = hipHostMalloc((void**)&hmem_, total, hipHostMallocCoherent);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&staged_, hipEventDisableTiming);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&run_done_, hipEventDisableTiming);
   if (e != hipSuccess) {
@@ -175,7 +177,7 @@ int BatchSlot::run(const LaunchConfig& cfg, hipStream_t stream, bool graph, bool
       if (mark_last_run() == hipSuccess) e = hipStreamWaitEvent(stream, run_done_, 0);
       else if ((rc = wait_done()) != MRG8_OK) return rc;  // the last stream no longer accepts work
     }
-    if (e == hipSuccess) e = hipMemcpyAsync(dmem_, hmem_, used_bytes_, hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess) e = launch_stage(hmem_, dmem_, used_bytes_, stream);
     if (e == hipSuccess) e = hipEventRecord(staged_, stream);
     if (e != hipSuccess) return hip_fail(e, "batch staging copy");
     copy_pending_ = true;

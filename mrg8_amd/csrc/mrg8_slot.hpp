@@ -2,9 +2,10 @@
 //
 // One device block holds the batch block of mrg8_device.hpp (header,
 // segments, tile prefix sums, tile -> segment map), sized for `cap` segments
-// and `max_tiles` tiles; a pinned host block of the same size stages it.  A
-// run copies the used prefix host-to-device in one copy when it changed since
-// the last run, on the launch stream, and launches the kernel directly or
+// and `max_tiles` tiles; a coherent pinned host block of the same size
+// stages it.  A run copies the used prefix to the device with a small kernel
+// (k_stage) when it changed since the last run, on the launch stream, and
+// launches the kernel directly or
 // replays a captured graph holding it; the kernel reads the counts and table
 // offsets from the header, so a graph is captured once per launch
 // configuration.
