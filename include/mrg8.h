@@ -114,21 +114,28 @@ int mrg8_advance_state_bits(const uint32_t state[8], const short bits[200], uint
  * (cuda/mrg8_cuda.cu:282-399,201-280 via :402-436) with no size limit
  * (n is 64-bit; the reference takes int and jumps only 32 bits on device). */
 int mrg8_generate(mrg8_gen_t g, void* dptr, uint64_t n, int format, void* stream);
-/* Same, with an explicit launch shape: threads per block (64..1024, multiple
- * of 64), outputs per wave (at most n, rounded up to 64 * outputs per lane)
- * and outputs per lane (1, 2 or 4); 0 means automatic for each.  The output is
- * identical for every shape. */
+/* Same, with an explicit launch shape; 0 means automatic for each, and the
+ * output is identical for every shape.  outputs_per_lane 1, 2 or 4 selects the
+ * vector kernel (threads per block 64..1024, a multiple of 64; outputs per
+ * wave at most n, rounded up to 64 * outputs per lane); 16 or 32 selects the
+ * matrix-core fill (int8 MFMAs; tiles of 16 columns of 4 * outputs_per_lane
+ * outputs, threads per block 256, 512 or 1024, outputs per wave a multiple of
+ * the tile), which falls back to the vector kernel when the fill holds no
+ * whole tile or the state is all zero.  Automatic: the matrix-core fill for
+ * word fills of 2^22 outputs or more, the vector kernel otherwise, unless
+ * mrg8_tune_fill chose a shape for that size class. */
 int mrg8_generate_shape(mrg8_gen_t g, void* dptr, uint64_t n, int format, void* stream,
                         int block_threads, uint64_t outputs_per_wave, int outputs_per_lane);
 /* Launch shape of a single-stream fill. */
 typedef struct mrg8_fill_shape {
   int32_t block_threads;      /* threads per block */
-  int32_t outputs_per_lane;   /* outputs each lane makes per iteration */
+  int32_t outputs_per_lane;   /* 1, 2, 4: vector kernel; 16, 32: matrix-core fill */
   uint64_t outputs_per_wave;  /* consecutive outputs per wave (the jump stride between waves) */
 } mrg8_fill_shape;
 
 /* Autotuner for single-stream fills: times threads-per-block x outputs-per-
- * lane x outputs-per-wave on `device` for fills of about n outputs in
+ * lane (vector kernel and matrix-core fill) x outputs-per-wave on `device` for
+ * fills of about n outputs in
  * `format` (a scratch buffer of n outputs is allocated for the trials) and
  * makes the fastest the default shape of mrg8_generate for that format and
  * size class (floor(log2 n)) on that device.  `cache_path` (may be NULL) is a

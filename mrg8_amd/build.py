@@ -26,8 +26,12 @@ COMMON = ["-O3", "-fPIC", "-std=c++17", "-Wall", "-Wextra", "-Wno-unused-paramet
           "-I" + os.path.join(ROOT, "include")]
 # Device code: gfx950 only; packed FP32 instructions are disabled (they are
 # not needed here and some forms have proven unreliable on this hardware).
+# MFMA results go to ordinary VGPRs: the matrix-core fill reads every result
+# with VALU instructions, and results left in AGPRs cost one v_accvgpr_read
+# per value (four per output) -- 189 VGPRs either way fit its two waves per SIMD.
 DEVICE = ["-x", "hip", "--offload-arch=" + ARCH,
-          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+          "-mllvm", "-amdgpu-mfma-vgpr-form"]
 
 
 def hipcc() -> str:

@@ -104,11 +104,28 @@ int device_context(int device, DeviceContext** out) {
     jump[(size_t)kIdentityRec + r] = make_record(row);
   }
 
+  // Matrix-core fill tables (operand digits, combine constants, column
+  // start records); a fan-out row with a zero entry would need the exact-zero
+  // test, so such a table turns the matrix-core fill off.
+  MfmaHostTables mt[2];
+  MfmaStrideTables mn[2];
+  bool mfma_ok = true;
+  for (int i = 0; i < 2; ++i)
+    mfma_ok = mfma_ok && mfma_host_tables(kMfmaColumns, kMfmaColumnLen[i], kMfmaClasses, kMfmaBeta, &mt[i]) &&
+              mfma_stride_tables(host_jump_table(), (uint64_t)kMfmaColumns * kMfmaColumnLen[i], kMfmaClasses, kMfmaBeta,
+                                 &mn[i]);
+
   auto ctx = std::make_unique<DeviceContext>();
   ctx->device = device;
   ctx->num_cu = prop.multiProcessorCount;
   void* p = nullptr;
-  const size_t bytes = sizeof(CoefRecord) * (lane.size() + jump.size());
+  const size_t rec_bytes = sizeof(CoefRecord) * (lane.size() + jump.size());
+  // operand digits and accumulator start values do not depend on the column length
+  size_t mfma_bytes = sizeof(uint32_t) * (mt[0].a.size() + mt[0].c.size());
+  for (int i = 0; i < 2; ++i)
+    mfma_bytes += sizeof(CoefRecord) * mt[i].fan.size() + sizeof(uint32_t) * (mn[i].a.size() + mn[i].k.size());
+  if (!mfma_ok) mfma_bytes = 0;
+  const size_t bytes = rec_bytes + mfma_bytes;
   if ((e = hipMalloc(&p, bytes)) != hipSuccess) {
     (void)hipSetDevice(prev);
     return hip_fail(e, "hipMalloc(tables)");
@@ -118,6 +135,28 @@ int device_context(int device, DeviceContext** out) {
   e = hipMemcpy(ctx->tables.lane, lane.data(), sizeof(CoefRecord) * lane.size(), hipMemcpyHostToDevice);
   if (e == hipSuccess)
     e = hipMemcpy(ctx->tables.jump, jump.data(), sizeof(CoefRecord) * jump.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess && mfma_ok) {
+    // records first (16-byte aligned after the 48-byte records), then the
+    // operand digits, the accumulator start values and the step tables
+    uint8_t* m = static_cast<uint8_t*>(p) + rec_bytes;
+    auto put = [&](const void* src, size_t n) {
+      uint8_t* at = m;
+      if (e == hipSuccess) e = hipMemcpy(at, src, n, hipMemcpyHostToDevice);
+      m += n;
+      return at;
+    };
+    for (int i = 0; i < 2; ++i)
+      ctx->tables.mfma_fan[i] =
+          reinterpret_cast<const CoefRecord*>(put(mt[i].fan.data(), sizeof(CoefRecord) * mt[i].fan.size()));
+    ctx->tables.mfma_a = reinterpret_cast<const uint32_t*>(put(mt[0].a.data(), sizeof(uint32_t) * mt[0].a.size()));
+    ctx->tables.mfma_c = reinterpret_cast<const int32_t*>(put(mt[0].c.data(), sizeof(int32_t) * mt[0].c.size()));
+    for (int i = 0; i < 2; ++i) {
+      ctx->tables.mfma_next_a[i] =
+          reinterpret_cast<const uint32_t*>(put(mn[i].a.data(), sizeof(uint32_t) * mn[i].a.size()));
+      ctx->tables.mfma_next_k[i] =
+          reinterpret_cast<const uint32_t*>(put(mn[i].k.data(), sizeof(uint32_t) * mn[i].k.size()));
+    }
+  }
   if (e != hipSuccess) {
     (void)hipFree(p);
     (void)hipSetDevice(prev);
@@ -211,32 +250,109 @@ void default_shape(const DeviceContext& ctx, uint64_t n, int format, int per_lan
   *block = b;
 }
 
+namespace {
+
+// Matrix-core fill of n outputs: whole tiles of 16 x `column` outputs from
+// the first 1 KiB boundary of dptr on, `outputs_per_wave` / tile consecutive
+// tiles per wave (0: 32 KiB of output) in blocks of `block_threads` (0:
+// the format's default), the outputs before the first tile and after the last
+// by two extra waves.  Returns MRG8_ERR_STATE (and launches nothing) when the
+// fill has no whole tile or the state is all zero: the caller takes the
+// vector kernel.
+int fill_mfma(DeviceContext& ctx, const uint32_t state[8], uint64_t offset, void* dptr, uint64_t n, int format,
+              hipStream_t stream, int column, int block_threads, uint64_t outputs_per_wave) {
+  const int esz = format_size(format);
+  uint32_t any = 0;
+  for (int j = 0; j < 8; ++j) any |= state[j];
+  if (!ctx.tables.mfma_a || any == 0) return MRG8_ERR_STATE;
+  const uint64_t align = 1024;
+  uint64_t head = ((align - (reinterpret_cast<uintptr_t>(dptr) & (align - 1))) & (align - 1)) / (uint64_t)esz;
+  if (head >= n) return MRG8_ERR_STATE;
+  const uint64_t tile = (uint64_t)kMfmaColumns * column;
+  const uint64_t tiles = (n - head) / tile;
+  if (tiles == 0) return MRG8_ERR_STATE;
+  const int ci = column == 64 ? 0 : 1;
+  MfmaFillArgs a{};
+  a.out = dptr;
+  a.head = head;
+  a.tiles = tiles;
+  a.column = (uint32_t)column;
+  a.waves_per_block = (uint32_t)(block_threads ? block_threads / 64 : format == MRG8_U32 ? 8 : 16);
+  // default: 32 KiB of output per wave, fewer tiles when that would leave
+  // fewer than 8 waves per CU
+  const uint64_t per = outputs_per_wave
+                           ? outputs_per_wave / tile
+                           : std::min<uint64_t>(32768 / (uint64_t)esz / tile, tiles / ((uint64_t)ctx.num_cu * 8));
+  a.tiles_per_wave = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(per, 1), 1u << 20);
+  a.waves = (tiles + a.tiles_per_wave - 1) / a.tiles_per_wave;
+  const uint64_t tail_first = head + tiles * tile;
+  if (head) {
+    a.extra_first[a.extra] = 0;
+    a.extra_count[a.extra++] = head;
+  }
+  if (tail_first < n) {
+    a.extra_first[a.extra] = tail_first;
+    a.extra_count[a.extra++] = n - tail_first;
+  }
+  std::memcpy(a.start.state, state, sizeof a.start.state);
+  a.start.offset = offset;
+  a.lane = ctx.tables.lane;
+  a.jump = ctx.tables.jump;
+  a.mfma_a = ctx.tables.mfma_a;
+  a.mfma_c = ctx.tables.mfma_c;
+  a.mfma_fan = ctx.tables.mfma_fan[ci];
+  a.mfma_next_a = ctx.tables.mfma_next_a[ci];
+  a.mfma_next_k = ctx.tables.mfma_next_k[ci];
+  DeviceGuard dg(ctx.device);
+  const hipError_t e = launch_fill_mfma(a, format, stream);
+  if (e != hipSuccess) return hip_fail(e, "matrix-core fill launch");
+  return MRG8_OK;
+}
+
+}  // namespace
+
 int fill_device(DeviceContext& ctx, const uint32_t state[8], uint64_t offset, void* dptr, uint64_t n, int format,
                 hipStream_t stream, int block_threads, uint64_t tile, int per_lane) {
   if (n == 0) return MRG8_OK;
   const int esz = format_size(format);
-  // 4 outputs per lane (16-byte stores) whatever the destination's alignment:
-  // each wave writes the outputs before its first 1 KiB boundary one per lane.
-  int pl = 4;
-  if (per_lane) {
-    if (per_lane != 1 && per_lane != 2 && per_lane != 4) return fail(MRG8_ERR_ARG, "outputs_per_lane must be 1, 2 or 4");
-    pl = per_lane;
-  }
-  int def_block = 0;
-  uint64_t def_tile = 0;
   const int cls = 63 - __builtin_clzll(n);
   TunedShape tuned;
   {
     std::lock_guard<std::mutex> lk(ctx.tuned_mu);
     tuned = ctx.tuned[format][cls];
   }
-  if (tuned.valid && !per_lane) {
-    pl = tuned.per_lane;
-    def_block = tuned.block_threads;
-    def_tile = tuned.tile;
-  } else {
-    default_shape(ctx, n, format, pl, &def_block, &def_tile);
+  // The shape: the caller's, else the tuned one for this size class, else
+  // the default (the matrix-core fill for large word fills, the vector
+  // kernel otherwise).
+  int pl = per_lane;
+  if (!pl) {
+    if (tuned.valid) {
+      pl = tuned.per_lane;
+      if (block_threads <= 0) block_threads = tuned.block_threads;
+      if (tile == 0) tile = tuned.tile;
+    } else if (format == MRG8_U32 && n >= kMfmaDefaultMinWords && block_threads <= 0 && tile == 0) {
+      pl = 32;
+    }
   }
+  if (pl == 16 || pl == 32) {  // the matrix-core fill, columns of 4 pl outputs
+    const int column = 4 * pl;
+    if (block_threads > 0 && block_threads != 256 && block_threads != 512 && block_threads != 1024)
+      return fail(MRG8_ERR_ARG, "the matrix-core fill takes blocks of 256, 512 or 1024 threads");
+    if (tile % ((uint64_t)kMfmaColumns * column))
+      return fail(MRG8_ERR_ARG, "the matrix-core fill takes whole %d-output tiles per wave", kMfmaColumns * column);
+    const int rc = fill_mfma(ctx, state, offset, dptr, n, format, stream, column, std::max(block_threads, 0), tile);
+    if (rc != MRG8_ERR_STATE) return rc;
+    pl = 0;  // too short for a tile, or the all-zero state: the vector kernel's default shape
+    block_threads = 0;
+    tile = 0;
+  }
+  // 4 outputs per lane (16-byte stores) whatever the destination's alignment:
+  // each wave writes the outputs before its first 1 KiB boundary one per lane.
+  if (pl && pl != 1 && pl != 2 && pl != 4) return fail(MRG8_ERR_ARG, "outputs_per_lane must be 1, 2, 4, 16 or 32");
+  if (!pl) pl = 4;
+  int def_block = 0;
+  uint64_t def_tile = 0;
+  default_shape(ctx, n, format, pl, &def_block, &def_tile);
   if (block_threads <= 0) block_threads = def_block;
   if (block_threads < 64 || block_threads > 1024 || block_threads % 64)
     return fail(MRG8_ERR_ARG, "block_threads %d must be a multiple of 64 in [64, 1024]", block_threads);
@@ -784,7 +900,8 @@ extern "C" int mrg8_tune_fill(int device, int format, uint64_t n, const char* ca
       unsigned long long tile = 0;
       double ms = 0;
       while (std::fscanf(f, "%255s %d %d %llu %lf", k, &bt, &pl, &tile, &ms) == 5)
-        if (key == k && bt >= 64 && bt <= 1024 && bt % 64 == 0 && (pl == 1 || pl == 2 || pl == 4) && tile) {
+        if (key == k && bt >= 64 && bt <= 1024 && bt % 64 == 0 && (pl == 1 || pl == 2 || pl == 4 || pl == 16 || pl == 32) &&
+            tile) {
           found.valid = true;
           found.block_threads = bt;
           found.per_lane = pl;
@@ -812,11 +929,13 @@ extern "C" int mrg8_tune_fill(int device, int format, uint64_t n, const char* ca
     double best_ms = 1e30;
     const int blocks[] = {256, 512, 1024};
     const uint64_t tiles[] = {2048, 4096, 8192, 16384};
-    const int pls[] = {1, 2, 4};
+    // vector kernel: 1, 2 or 4 outputs per lane per iteration; matrix-core
+    // fill: 16 or 32 outputs per lane per tile (columns of 64 or 128)
+    const int pls[] = {1, 2, 4, 16, 32};
     for (int pl : pls) {
       for (int bt : blocks)
         for (uint64_t tile : tiles) {
-          if (tile < 64ull * pl * 8) continue;
+          if (pl <= 4 ? tile < 64ull * pl * 8 : (bt == 256 || tile % (64ull * pl) || tile * esz > 65536)) continue;
           rc = fill_device(*ctx, st, 0, buf, n, format, nullptr, bt, tile, pl);  // warm-up
           if (rc) break;
           (void)hipEventRecord(e0, nullptr);

@@ -709,6 +709,227 @@ __global__ __launch_bounds__(256) void k_stage(const u4_t* __restrict__ src, u4_
     reinterpret_cast<uint32_t*>(dst + n16)[i] = reinterpret_cast<const uint32_t*>(src + n16)[i];
 }
 
+// ---- matrix-core fill ---------------------------------------------------------
+//
+// A tile of 16 COL consecutive outputs is cut into 16 columns of COL (64
+// or 128); column c starts from its own state (its 8 preceding words)
+// and one step makes its next 16 outputs.  Output j of a step (j < 16) is
+// dot(R_j, w) for the column's state w (oldest word first), with R_j the
+// block rows.  In exact integers mod p:
+//   * each state word is centred, e = w - 2^30, and cut into four balanced
+//     base-256 digits e = sum_b 2^(8b) d_b (d_b in [-128, 127]);
+//   * M_b = 2^(8b) R_j mod p (centred) is cut the same way,
+//     M_b = sum_a 2^(8a) m_(b,a), so that
+//     dot(R_j, e) == sum_a 2^(8a) sum_(q,b) m_(b,a)[q] d_b[q]  (mod p):
+//     four int8 products of K = 8 words x 4 digits = 32, one per class a,
+//     v_mfma_i32_16x16x32_i8 with the step's rows as M, the columns as N;
+//   * the accumulators start from per-row constants (the C operand) that
+//     carry the 2^30 sum R_j, the reference's -1 and the biases, so that
+//     t = C_0 + 2^8 C_1 and u = C_2 + 2^8 C_3 are non-negative and below
+//     2^29, and x = t + ((u << 16) & p) + (u >> 15) (2^16 u mod p is a 31-bit
+//     rotation) is the residue plus at most one p: eight 32-bit operations
+//     per output.
+// MFMA layouts (gfx950, checked on the device): lane l = n + 16 g holds the
+// 8 bytes of K-group g for row/column n, and result register i of lane l is
+// row 4 g + i of column n.  K-group g, byte i is word W(g) + i / 4, digit
+// i % 4, with W = (0, 4, 2, 6); row 4 g + i is step position 2 g + i for
+// i < 2 and 8 + W(g) + i - 2 for i >= 2.  So the step's last eight outputs
+// (positions 8..15, the column's next state) are registers 2 and 3 of
+// exactly the lanes whose K-group needs them: no lane exchange.
+// A wave makes a few consecutive tiles (32 KiB of output: the store
+// pattern of short-lived waves in address order, which persistent waves
+// striding over the tiles did not match: 427-474 against 339 us for 2^28
+// doubles).  The 16 column states of the wave's next tile are the current
+// ones times A^2048: one more product of the same form, whose rows put the
+// new words in registers 0 and 1 of the lanes that need them.  Only a
+// wave's first tile computes its column states from the tile state (the
+// fan-out records).  Registers stay below 128 and the tile's staging below
+// 8.5 KiB, so four waves per SIMD are resident.
+// The outputs of a tile are staged in LDS (16 columns x 132 words per wave)
+// and leave as the vector kernel's 1 KiB wave stores in address order.
+
+typedef int32_t v4i_t __attribute__((ext_vector_type(4)));
+
+
+// Balanced base-256 digits of w - 2^30 (w < 2^31), one per byte.
+__device__ __forceinline__ uint32_t mfma_digits(uint32_t w) { return (w + 0x40808080u) ^ 0x80808080u; }
+
+__device__ __forceinline__ long mfma_operand(uint32_t w0, uint32_t w1) {
+  return (long)((uint64_t)mfma_digits(w0) | ((uint64_t)mfma_digits(w1) << 32));
+}
+
+__device__ __forceinline__ uint64_t mad_u64_u32(uint32_t a, uint32_t b, uint64_t c) {
+  uint64_t w, carry;
+  asm("v_mad_u64_u32 %0, %1, %2, %3, %4" : "=v"(w), "=s"(carry) : "v"(a), "s"(b), "v"(c));
+  return w;
+}
+
+// Residue in [0, p) of a step output (result register i of the four
+// classes, accumulators started from the step constants).
+__device__ __forceinline__ uint32_t mfma_step_residue(const v4i_t (&acc)[kMfmaClasses], int i) {
+  const uint32_t t = (uint32_t)acc[0][i] + ((uint32_t)acc[1][i] << 8);  // [0, 2 beta + 2^16)
+  const uint32_t u = (uint32_t)acc[2][i] + ((uint32_t)acc[3][i] << 8);  // [0, 2 beta + 2^15)
+  const uint32_t x = t + ((u << 16) & kP) + (u >> 15);                  // < 2 p
+  return min(x, x - kP);
+}
+
+// Residue in [0, p) of a stride output (accumulators started from 0, k in
+// [kMfmaBeta, kMfmaBeta + p)).
+__device__ __forceinline__ uint32_t mfma_stride_residue(const v4i_t (&acc)[kMfmaClasses], int i, uint32_t k) {
+  const uint32_t t = (uint32_t)acc[0][i] + ((uint32_t)acc[1][i] << 8);              // [-beta, beta]
+  const uint32_t u = (uint32_t)acc[2][i] + ((uint32_t)acc[3][i] << 8) + kMfmaBeta;  // [0, 2 beta]
+  const uint64_t y = mad_u64_u32(u, 65536u, (uint64_t)(t + k));                    // < 2^45
+  const uint32_t x = ((uint32_t)y & kP) + (uint32_t)(y >> 31);                      // < 2^31 + 2^14
+  return min(x, x - kP);
+}
+
+// The tiles [t0, t0 + nt) of a matrix-core fill with columns of COL outputs,
+// made by one wave with the LDS area st (kMfmaColumns * (COL + 4) words: the
+// 4-word pad spreads the columns over the LDS banks).
+template <int FMT, int COL>
+__device__ __forceinline__ void mfma_tiles(const MfmaFillArgs& a, uint64_t t0, uint32_t nt, uint32_t* st) {
+  typedef typename Out<FMT>::T T;
+  constexpr int kTile = kMfmaColumns * COL;
+  constexpr int kStride = COL + 4;  // LDS words per column
+  constexpr int kSteps = COL / 16;
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t n = lane & 15, g = lane >> 4;
+  const uint32_t wg = (g & 1) * 4 + (g >> 1) * 2;  // W(g): first word of this K-group
+  long A[kMfmaClasses];
+  v4i_t C0[kMfmaClasses];
+#pragma unroll
+  for (int q = 0; q < kMfmaClasses; ++q) {
+    A[q] = reinterpret_cast<const long*>(a.mfma_a)[q * 64 + lane];
+    C0[q] = reinterpret_cast<const v4i_t*>(a.mfma_c)[q * 4 + g];
+  }
+  long AJ[kMfmaClasses] = {};
+  u2_t KJ = {};
+  if (nt > 1) {
+#pragma unroll
+    for (int q = 0; q < kMfmaClasses; ++q) AJ[q] = reinterpret_cast<const long*>(a.mfma_next_a)[q * 64 + lane];
+    KJ = reinterpret_cast<const u2_t*>(a.mfma_next_k)[g];
+  }
+  // the first tile's column states: words W(g), W(g) + 1 (oldest first) of
+  // column n are the tile's outputs 128 n - 8 + q, from the tile state
+  long B;
+  {
+    uint32_t s[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) s[j] = a.start.state[j];
+    jump_groups(s, a.start.offset + a.head + t0 * kTile, a.jump);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) s[j] = __builtin_amdgcn_readfirstlane(s[j]);
+    Coef k2[2];
+    const CoefRecord* fan = a.mfma_fan + (n ? n - 1 : 0) * 8 + wg;  // outputs COL n - 8 + q
+    k2[0] = load_coef(fan);
+    k2[1] = load_coef(fan + 1);
+    int32_t e[8];
+    centre(s, e);
+    uint32_t r[2];
+    dot_residues<2>(k2, e, r);
+    // column 0 starts from the tile state itself: word q is s[7 - q]
+    uint32_t s0 = s[7], s1 = s[6];
+#pragma unroll
+    for (int q = 2; q < 8; q += 2) {
+      s0 = wg == (uint32_t)q ? s[7 - q] : s0;
+      s1 = wg == (uint32_t)q ? s[6 - q] : s1;
+    }
+    B = mfma_operand(n ? r[0] + 1u : s0, n ? r[1] + 1u : s1);
+  }
+
+  // A tile's outputs go through LDS into this lane's slice of its 1 KiB wave
+  // stores (pend: 32 residues, chunk ch = this lane's part of store ch), and
+  // the stores leave during the next tile's steps, kChunks / kSteps per step,
+  // so that every wave keeps writing while it computes.
+  constexpr int kChunks = kTile * (int)sizeof(T) / 1024;  // 1 KiB stores per tile
+  constexpr int kPer = 1024 / (64 * (int)sizeof(T));      // outputs per lane per store
+  static_assert(kChunks % kSteps == 0 && COL % kPer == 0, "tile shape");
+  uint32_t pend[kChunks * kPer];
+  T* pend_base = nullptr;
+  auto store_chunk = [&](int ch) {
+    if constexpr (FMT == 2) {
+      const u4_t o = {pend[4 * ch] + 1u, pend[4 * ch + 1] + 1u, pend[4 * ch + 2] + 1u, pend[4 * ch + 3] + 1u};
+      __builtin_nontemporal_store(o, reinterpret_cast<u4_t*>(pend_base + 256 * ch + 4 * lane));
+    } else {
+      const d2_t o = {convert<FMT>(pend[2 * ch]), convert<FMT>(pend[2 * ch + 1])};
+      __builtin_nontemporal_store(o, reinterpret_cast<d2_t*>(pend_base + 128 * ch + 2 * lane));
+    }
+  };
+  uint32_t* const col = st + n * kStride;
+  for (uint32_t t = 0; t < nt; ++t) {
+    const long B0 = B;
+#pragma unroll
+    for (int step = 0; step < kSteps; ++step) {
+      v4i_t acc[kMfmaClasses];
+#pragma unroll
+      for (int q = 0; q < kMfmaClasses; ++q) acc[q] = __builtin_amdgcn_mfma_i32_16x16x32_i8(A[q], B, C0[q], 0, 0, 0);
+      uint32_t r[4];
+#pragma unroll
+      for (int i = 3; i >= 0; --i) r[i] = mfma_step_residue(acc, i);  // registers 2, 3 first: the next operand
+      if (step + 1 < kSteps) B = mfma_operand(r[2] + 1u, r[3] + 1u);
+      *reinterpret_cast<u2_t*>(col + 16 * step + 2 * g) = u2_t{r[0], r[1]};
+      *reinterpret_cast<u2_t*>(col + 16 * step + 8 + wg) = u2_t{r[2], r[3]};
+      if (t) {
+#pragma unroll
+        for (int k = 0; k < kChunks / kSteps; ++k) store_chunk(step * (kChunks / kSteps) + k);
+      }
+    }
+    // the next tile's column states: A^(16 COL) times this tile's (register
+    // i < 2 of this lane: word W(g) + i)
+    if (t + 1 < nt) {
+      v4i_t acc[kMfmaClasses];
+#pragma unroll
+      for (int q = 0; q < kMfmaClasses; ++q)
+        acc[q] = __builtin_amdgcn_mfma_i32_16x16x32_i8(AJ[q], B0, v4i_t{}, 0, 0, 0);
+      B = mfma_operand(mfma_stride_residue(acc, 0, KJ.x) + 1u, mfma_stride_residue(acc, 1, KJ.y) + 1u);
+    }
+    // this tile's slice of every store: tile outputs ch E + kPer lane .. + kPer - 1
+    // (E = 1024 / sizeof(T) per store), within one column
+#pragma unroll
+    for (int ch = 0; ch < kChunks; ++ch) {
+      const uint32_t pos = ch * (1024 / (int)sizeof(T)) + kPer * lane;
+      const uint32_t* src = st + (pos / COL) * kStride + pos % COL;
+      if constexpr (FMT == 2) {
+        const u4_t v = *reinterpret_cast<const u4_t*>(src);
+        pend[4 * ch] = v.x; pend[4 * ch + 1] = v.y; pend[4 * ch + 2] = v.z; pend[4 * ch + 3] = v.w;
+      } else {
+        const u2_t v = *reinterpret_cast<const u2_t*>(src);
+        pend[2 * ch] = v.x; pend[2 * ch + 1] = v.y;
+      }
+    }
+    pend_base = static_cast<T*>(a.out) + a.head + (t0 + t) * kTile;
+  }
+#pragma unroll
+  for (int ch = 0; ch < kChunks; ++ch) store_chunk(ch);
+}
+
+// Matrix-core fill: waves [0, a.waves) make tiles, the next a.extra waves
+// the short runs before the first tile and after the last with the vector
+// kernel's loop.
+template <int FMT, int COL, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void k_fill_mfma(MfmaFillArgs a) {
+  __shared__ uint32_t stage[WAVES][kMfmaColumns * (COL + 4)];
+  const uint32_t wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const uint64_t w = (uint64_t)blockIdx.x * WAVES + wib;
+  if (w < a.waves) {
+    const uint64_t t0 = w * a.tiles_per_wave;
+    mfma_tiles<FMT, COL>(a, t0, (uint32_t)min((uint64_t)a.tiles_per_wave, a.tiles - t0), stage[wib]);
+    return;
+  }
+  const uint64_t e = w - a.waves;
+  if (e >= a.extra) return;
+  const uint64_t first = a.extra_first[e], cnt = a.extra_count[e];
+  uint32_t s[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) s[j] = a.start.state[j];
+  jump_groups(s, a.start.offset + first, a.jump);
+#pragma unroll
+  for (int j = 0; j < 8; ++j) s[j] = __builtin_amdgcn_readfirstlane(s[j]);
+  Coef k[4];
+  load_lane_coefs<FMT, 4>(k, a.lane);
+  generate_run<FMT, 4>(static_cast<typename Out<FMT>::T*>(a.out) + first, cnt, s, k, a.lane);
+}
+
 template <int FMT, int PL>
 hipError_t fill_one(const FillArgs& a, int block_threads, hipStream_t stream) {
   if (a.n == 0) return hipSuccess;
@@ -733,6 +954,47 @@ hipError_t launch_fill(const FillArgs& a, int format, int per_lane, int block_th
     case 2 * 8 + 1: return fill_one<2, 1>(a, block_threads, stream);
     case 2 * 8 + 2: return fill_one<2, 2>(a, block_threads, stream);
     case 2 * 8 + 4: return fill_one<2, 4>(a, block_threads, stream);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+namespace {
+
+template <int FMT, int COL, int WAVES>
+hipError_t fill_mfma_one(const MfmaFillArgs& a, uint64_t blocks, hipStream_t stream) {
+  hipLaunchKernelGGL((k_fill_mfma<FMT, COL, WAVES>), dim3((uint32_t)blocks), dim3(64 * WAVES), 0, stream, a);
+  return hipGetLastError();
+}
+
+template <int FMT, int COL>
+hipError_t fill_mfma_waves(const MfmaFillArgs& a, uint64_t blocks, hipStream_t stream) {
+  switch (a.waves_per_block) {
+    case 4: return fill_mfma_one<FMT, COL, 4>(a, blocks, stream);
+    case 8: return fill_mfma_one<FMT, COL, 8>(a, blocks, stream);
+    case 16: return fill_mfma_one<FMT, COL, 16>(a, blocks, stream);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+template <int FMT>
+hipError_t fill_mfma_column(const MfmaFillArgs& a, uint64_t blocks, hipStream_t stream) {
+  return a.column == 64 ? fill_mfma_waves<FMT, 64>(a, blocks, stream) : fill_mfma_waves<FMT, 128>(a, blocks, stream);
+}
+
+}  // namespace
+
+hipError_t launch_fill_mfma(const MfmaFillArgs& a, int format, hipStream_t stream) {
+  const uint64_t waves = a.waves + a.extra;
+  if (waves == 0) return hipSuccess;
+  if (a.extra > 2 || a.tiles_per_wave == 0 || a.waves != (a.tiles + a.tiles_per_wave - 1) / a.tiles_per_wave ||
+      (a.column != 64 && a.column != 128) || (a.waves_per_block != 4 && a.waves_per_block != 8 && a.waves_per_block != 16))
+    return hipErrorInvalidValue;
+  const uint64_t blocks = (waves + a.waves_per_block - 1) / a.waves_per_block;
+  if (blocks * 64 * a.waves_per_block > 0xffffffffull) return hipErrorInvalidConfiguration;  // 32-bit grid
+  switch (format) {
+    case 0: return fill_mfma_column<0>(a, blocks, stream);
+    case 1: return fill_mfma_column<1>(a, blocks, stream);
+    case 2: return fill_mfma_column<2>(a, blocks, stream);
     default: return hipErrorInvalidValue;
   }
 }

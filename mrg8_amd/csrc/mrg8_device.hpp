@@ -39,7 +39,32 @@ struct DeviceTables {
   // (((k - kJumpPositions) * kJumpValues) + (d - 1)) * 8 + r; null until the
   // first bulk advance with offsets wider than 64 bits
   CoefRecord* wide = nullptr;
+  // Matrix-core fill (k_fill_mfma): kMfmaClasses x 64 lanes x 8 bytes of
+  // operand digits and kMfmaClasses x 4 x 4 accumulator start values; per
+  // column length kMfmaColumnLen[i], the (kMfmaColumns - 1) x 8 records that
+  // give a wave's first column states, and A^(tile outputs) in operand form
+  // (kMfmaClasses x 64 x 2 words) with its combine constants (4 x 2): the
+  // column states of a wave's next tile.
+  const uint32_t* mfma_a = nullptr;
+  const int32_t* mfma_c = nullptr;
+  const CoefRecord* mfma_fan[2] = {nullptr, nullptr};
+  const uint32_t* mfma_next_a[2] = {nullptr, nullptr};
+  const uint32_t* mfma_next_k[2] = {nullptr, nullptr};
 };
+
+// Matrix-core fill.  A tile is kMfmaColumns columns of `column` consecutive
+// outputs (kMfmaColumnLen: 64 or 128); one step of a wave makes 16
+// consecutive outputs of each column with int8 MFMAs (see k_fill_mfma).  The
+// shape value outputs_per_lane = column / 4 (16 or 32, the outputs each lane
+// makes per tile) selects this fill.
+constexpr int kMfmaColumns = 16;
+constexpr int kMfmaColumnLen[2] = {64, 128};
+constexpr int kMfmaClasses = 4;   // byte digits of a coefficient
+// Word fills of at least this many outputs take the matrix-core fill by
+// default (measured on MI355X below).
+constexpr uint64_t kMfmaDefaultMinWords = 1ull << 22;
+// Bias that keeps the combine's partial sums non-negative: |partial| <= 2^27 + 2^19.
+constexpr uint32_t kMfmaBeta = (1u << 27) + (1u << 19);
 
 // Wave-uniform starting point of a launch.
 struct StreamStart {
@@ -59,6 +84,34 @@ struct FillArgs {
   StreamStart start;        // the state, and the position (offset) of out[0]
   const CoefRecord* lane;
   const CoefRecord* jump;
+};
+
+// A matrix-core fill: `tiles` whole tiles of 16 x `column` outputs at
+// out[head + t 16 column] (1 KiB aligned); wave w makes tiles
+// w k .. w k + k - 1 (k = tiles_per_wave, the last wave fewer), moving its
+// column states from one tile to the next with the device's step tables
+// (A^(16 column) in operand form).  Up to two short runs (the outputs
+// before the first tile and after the last) are made by extra waves with the
+// vector kernel's loop.
+struct MfmaFillArgs {
+  void* out;
+  uint64_t head;            // outputs before the first tile
+  uint64_t tiles;
+  uint64_t waves;           // ceil(tiles / tiles_per_wave)
+  uint32_t tiles_per_wave;
+  uint32_t column;          // outputs per column: 64 or 128
+  uint32_t extra;           // 0..2 short runs
+  uint32_t waves_per_block; // 4, 8 or 16
+  uint64_t extra_first[2];  // first output of each short run
+  uint64_t extra_count[2];
+  StreamStart start;
+  const CoefRecord* lane;
+  const CoefRecord* jump;
+  const uint32_t* mfma_a;
+  const int32_t* mfma_c;
+  const CoefRecord* mfma_fan;   // this column length's tables
+  const uint32_t* mfma_next_a;  // kMfmaClasses x 64 x 2 words
+  const uint32_t* mfma_next_k;  // 4 x 2
 };
 
 // Waves of a fill of n >= 1 outputs.
@@ -156,6 +209,8 @@ struct AdvanceArgs {
 
 // Launchers (mrg8_device.hip).  All are asynchronous on `stream`.
 hipError_t launch_fill(const FillArgs& a, int format, int per_lane, int block_threads, hipStream_t stream);
+// Blocks of a.waves_per_block waves.
+hipError_t launch_fill_mfma(const MfmaFillArgs& a, int format, hipStream_t stream);
 hipError_t launch_batch(const BatchArgs& a, int grid, int block_threads, hipStream_t stream);
 // One tile per wave: ceil(ntiles / waves per block) blocks.
 hipError_t launch_batch_tiles(const BatchArgs& a, uint64_t ntiles, int block_threads, hipStream_t stream);

@@ -187,4 +187,157 @@ inline bool all_nonzero(const uint32_t row[8]) {
   return true;
 }
 
+// ---- matrix-core fill tables ------------------------------------------------
+//
+// The int8 MFMA form of a step (16 outputs of each of 16 columns, output j
+// being dot(row_j, s) for the column's newest-first state s): with w_q =
+// s_(7-q) (oldest first) and R_j[q] = row_j[7-q], each state word is
+// centred (e = w - 2^30) and cut into balanced base-256 digits d_b
+// (e = sum_b 2^(8b) d_b, d_b in [-128, 127]); M_b[j][q] = 2^(8b) R_j[q] mod p,
+// centred, is cut the same way into digits m_(b,a).  Then
+//   dot(row_j, s) - 1 == sum_a 2^(8a) C_a[j] + kappa_j  (mod p),
+//   C_a[j] = sum_(q,b) m_(b,a)[j][q] d_b[q],  kappa_j = 2^30 sum_q R_j[q] - 1,
+// |C_a| <= 32 * 2^14 = 2^19.
+// Layout (v_mfma_i32_16x16x32_i8): lane l = n + 16 g holds the 8 bytes of
+// K-group g for row/column n, result register i of lane l is row 4 g + i of
+// column n.  K-group g, byte i is word q = W(g) + i / 4, digit b = i % 4,
+// with W = (0, 4, 2, 6); row 4 g + i is step position P(g, i): 2 g + i for
+// i < 2, 8 + W(g) + i - 2 for i >= 2.  So the step's last eight outputs
+// (positions 8..15, the column's next state) land in registers 2 and 3 of
+// exactly the lanes whose K-group needs them.
+//
+// Step constants: the MFMAs start from per-row accumulator values c_a (the
+// C operand), so that t = C_0 + 2^8 C_1 and u = C_2 + 2^8 C_3 come out in
+// [0, 2 beta + 2^16) and [0, 2 beta + 2^15) and t + 2^16 u == the word's
+// residue + kappa_j:  with D = kappa_j - 65537 beta (mod p),
+//   t0 = beta + (D & 0xffff), u0 = beta + (D >> 16),
+//   c_0 = t0 & 0xff, c_1 = t0 >> 8, c_2 = u0 & 0xff, c_3 = u0 >> 8.
+// The kernel then reduces x = t + ((u << 16) & p) + (u >> 15) (2^16 u mod p
+// is a 31-bit rotation of u), x < 2 p.
+//
+// Stride tables: A^steps in the same digit form, row 4 g + i (i < 2) giving
+// new word W(g) + i (oldest first), rows with i >= 2 zero, so that the new
+// words land in registers 0 and 1 of the lanes whose K-group needs them;
+// per lane group g and register i < 2, K in [beta, beta + p) with
+// K == kappa - 2^16 beta (mod p), for the combine t + K + 2^16 (u + beta) of
+// accumulators that start from 0.
+//
+// Fan-out records: the start state of column c >= 1 of a tile is the tile's
+// outputs L c - 8 .. L c - 1 (L outputs per column), row 0 of A^(k+1) for
+// k = L c - 8 + q.
+constexpr int kMfmaWordOf[4] = {0, 4, 2, 6};  // W(g)
+
+inline int mfma_position(int g, int i) { return i < 2 ? 2 * g + i : 8 + kMfmaWordOf[g] + i - 2; }
+
+struct MfmaHostTables {
+  std::vector<uint32_t> a;      // [class][lane][2] words of packed digits
+  std::vector<int32_t> c;       // [class][lane group g][register i]
+  std::vector<CoefRecord> fan;  // [(c - 1) * 8 + q]
+};
+
+struct MfmaStrideTables {
+  std::vector<uint32_t> a;  // [class][lane][2]
+  std::vector<uint32_t> k;  // [g][i], i < 2
+};
+
+inline uint32_t centred_digits(int32_t m) { return (uint32_t)m + 0x80808080u; }  // bytes e_a: digit e_a - 128
+
+// Packed digit bytes of class a for K-group g of the row vector R
+// (oldest-first words): byte 4 v + b is digit a of centred(2^(8b) R[W(g) + v] mod p).
+inline void mfma_operand(const uint32_t R[8], int g, int a, uint32_t out[2]) {
+  for (int v = 0; v < 2; ++v) {
+    uint32_t word = 0;
+    for (int b = 0; b < 4; ++b) {
+      const uint32_t m = mul_mod(R[kMfmaWordOf[g] + v] % kP, 1u << (8 * b));
+      const int32_t mc = m <= (kP - 1) / 2 ? (int32_t)m : (int32_t)((int64_t)m - kP);
+      const uint32_t byte = ((centred_digits(mc) >> (8 * a)) & 0xffu) ^ 0x80u;
+      word |= byte << (8 * b);
+    }
+    out[v] = word;
+  }
+}
+
+inline uint32_t mfma_kappa(const uint32_t R[8]) {
+  uint32_t sum = 0;
+  for (int q = 0; q < 8; ++q) sum = add_mod(sum, R[q] % kP);
+  return add_mod(mul_mod(sum, 1u << 30), kP - 1);  // 2^30 sum - 1
+}
+
+inline bool mfma_host_tables(int columns, int column, int classes, uint32_t beta, MfmaHostTables* out) {
+  const std::vector<uint32_t> rows = block_rows(columns * column);
+  auto Rrow = [&](int j, uint32_t R[8]) {
+    for (int q = 0; q < 8; ++q) R[q] = rows[(size_t)j * 8 + 7 - q];
+  };
+  out->a.assign((size_t)classes * 64 * 2, 0);
+  for (int a = 0; a < classes; ++a)
+    for (int l = 0; l < 64; ++l) {
+      const int m = l & 15;  // row m = 4 g' + i' is position P(g', i')
+      uint32_t R[8];
+      Rrow(mfma_position(m >> 2, m & 3), R);
+      mfma_operand(R, l >> 4, a, &out->a[((size_t)a * 64 + l) * 2]);
+    }
+  out->c.assign((size_t)classes * 16, 0);
+  const uint32_t b65537 = mul_mod(mod_p(beta), 65537u);
+  for (int g = 0; g < 4; ++g)
+    for (int i = 0; i < 4; ++i) {
+      uint32_t R[8];
+      Rrow(mfma_position(g, i), R);
+      const uint32_t D = add_mod(mfma_kappa(R), kP - b65537);
+      const uint32_t t0 = beta + (D & 0xffffu), u0 = beta + (D >> 16);
+      const int32_t cv[4] = {(int32_t)(t0 & 0xffu), (int32_t)(t0 >> 8), (int32_t)(u0 & 0xffu), (int32_t)(u0 >> 8)};
+      for (int a = 0; a < classes; ++a) out->c[((size_t)a * 4 + g) * 4 + i] = cv[a];
+    }
+  out->fan.assign((size_t)(columns - 1) * 8, CoefRecord{});
+  for (int c = 1; c < columns; ++c)
+    for (int q = 0; q < 8; ++q) {
+      const uint32_t* row = &rows[((size_t)c * column - 8 + q) * 8];
+      if (!all_nonzero(row)) return false;
+      out->fan[(size_t)(c - 1) * 8 + q] = make_record(row);
+    }
+  for (int j = 0; j < 16; ++j)
+    if (!all_nonzero(&rows[(size_t)j * 8])) return false;
+  return true;
+}
+
+// A^steps (row-major, rows giving the newest-first words) through the table
+// of A^(2^b).
+inline std::vector<uint32_t> matrix_power(const JumpTable& jt, uint64_t steps) {
+  std::vector<uint32_t> m(64, 0);
+  for (int r = 0; r < 8; ++r) m[(size_t)r * 9] = 1;
+  for (int b = 0; b < 64; ++b)
+    if ((steps >> b) & 1u) m = mat_mul(jt.at(b), m.data());
+  return m;
+}
+
+// Stride tables for A^steps; false when a row has an entry == 0 (mod p),
+// whose exact-zero case the residue arithmetic does not carry.
+inline bool mfma_stride_tables(const JumpTable& jt, uint64_t steps, int classes, uint32_t beta,
+                               MfmaStrideTables* out) {
+  const std::vector<uint32_t> m = matrix_power(jt, steps);
+  for (int r = 0; r < 8; ++r)
+    if (!all_nonzero(&m[(size_t)r * 8])) return false;
+  // new word q (oldest first) = newest-first word 7 - q: R_q[q'] = m[7 - q][7 - q']
+  auto Rrow = [&](int q, uint32_t R[8]) {
+    for (int k = 0; k < 8; ++k) R[k] = m[(size_t)(7 - q) * 8 + 7 - k];
+  };
+  out->a.assign((size_t)classes * 64 * 2, 0);
+  for (int a = 0; a < classes; ++a)
+    for (int l = 0; l < 64; ++l) {
+      const int row = l & 15;
+      if ((row & 3) >= 2) continue;  // rows 4 g + 2, 4 g + 3: zero
+      uint32_t R[8];
+      Rrow(kMfmaWordOf[row >> 2] + (row & 3), R);
+      mfma_operand(R, l >> 4, a, &out->a[((size_t)a * 64 + l) * 2]);
+    }
+  out->k.assign(8, 0);
+  for (int g = 0; g < 4; ++g)
+    for (int i = 0; i < 2; ++i) {
+      uint32_t R[8];
+      Rrow(kMfmaWordOf[g] + i, R);
+      const uint32_t k = add_mod(mfma_kappa(R), kP - mul_mod(mod_p(beta), 1u << 16));  // kappa - 2^16 beta
+      out->k[(size_t)g * 2 + i] = k < beta ? k + kP : k;
+    }
+  return true;
+}
+
 }  // namespace mrg8

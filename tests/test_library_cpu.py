@@ -260,6 +260,21 @@ int main() {
     assert int(out.stdout) == GOLD["states_after"]["1000"][0]
 
 
+def test_mfma_emulation_matches_oracle():
+    """The matrix-core fill's arithmetic on the CPU: the library's MFMA tables
+    (operand digits, accumulator start values, fan-out records, step tables)
+    and a lane-by-lane replay of the kernel's int8 products, combine and
+    state hand-offs reproduce the oracle's words for columns of 64 and 128,
+    several strides and states with zero and p words
+    (tests/cpp/mfma_emul.cpp)."""
+    import subprocess
+
+    exe = os.path.join(ROOT, "tests", "cpp", "mfma_emul")
+    subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "tests", "cpp"), exe], check=True)
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0 and "mfma emulation ok" in r.stdout, r.stdout[-2000:]
+
+
 def test_torch_op_registered_and_rejects_cpu_tensors():
     import torch
 
