@@ -15,7 +15,9 @@ The same JSON line carries, measured in the same run (in this order):
                 the launch stream, one between every two launches; every
                 launch's duration listed) against 8 TB/s; HBM traffic from the
                 committed rocprofv3 counter summary when present
-  u32           the same workload in raw uint32 words
+  u32           the same workload in raw uint32 words (the matrix-core fill),
+                and with the vector kernel
+  f64_matrix_core  the float64 workload with the matrix-core fill
   small         2^24 float64 outputs (the reference's full-compare size)
   clock         the float64 fill from idle with the shader clock sampled after
                 each launch (the power manager's clamp), then the uint32 fill
@@ -156,8 +158,13 @@ def bench_single(args):
     buf = hip.DeviceBuffer(N_SINGLE * 8)
     g = mrg8_amd.mrg8_cuda(SEED)
     out = {"ceiling": write_ceiling(buf, N_SINGLE * 8, s)}
-    for fmt, name, esz in ((mrg8_amd.F64, "f64", 8), (mrg8_amd.U32, "u32", 4)):
-        step = lambda: g.generate(buf, N_SINGLE, fmt, stream=s)  # noqa: E731
+    # default shapes (doubles: the vector kernel; words: the matrix-core
+    # fill), then each format with the other kernel for comparison
+    legs = (("f64", mrg8_amd.F64, 8, {}), ("u32", mrg8_amd.U32, 4, {}),
+            ("u32_vector", mrg8_amd.U32, 4, {"per_lane": 4}),
+            ("f64_matrix_core", mrg8_amd.F64, 8, {"per_lane": 16, "block_threads": 1024, "tile": 2048}))
+    for name, fmt, esz, shape in legs:
+        step = lambda: g.generate(buf, N_SINGLE, fmt, stream=s, **shape)  # noqa: E731
         wall, km, per = timed_series(step, s, args.warmup, args.steps)
         out[name] = {"wall_s": wall, "kernel_ms": km, "per_launch_ms": per, "bytes": N_SINGLE * esz}
         log(f"single {name}: {N_SINGLE * args.steps / wall / 1e9:.1f} G samples/s wall, kernel {km:.3f} ms "
@@ -568,13 +575,24 @@ def main():
                      "valu_busy_pct": valu_busy_from_profiles("fill_f64_2e28_counters.txt", "k_fill<F64,4>"),
                      "algorithmic_bytes_per_launch": f64["bytes"]},
         "u32": {"gsps": round(N_SINGLE * args.steps / single["u32"]["wall_s"] / 1e9, 2),
+                "kernel": "k_fill_mfma<U32,128,8>: the matrix-core fill (int8 MFMA), the default for words; "
+                          "512-thread blocks, 8192 words per wave",
                 "kernel_ms": round(single["u32"]["kernel_ms"], 4),
+                "vector_kernel_ms": round(single["u32_vector"]["kernel_ms"], 4),
+                "vector_per_launch_ms": [round(x, 4) for x in single["u32_vector"]["per_launch_ms"]],
                 "gbs": round(single["u32"]["bytes"] / (single["u32"]["kernel_ms"] * 1e-3) / 1e9, 1),
                 "frac": round(single["u32"]["bytes"] / (single["u32"]["kernel_ms"] * 1e-3) / 1e9 / PEAK_GBS, 4),
                 "per_launch_ms": [round(x, 4) for x in single["u32"]["per_launch_ms"]],
                 "after_clock_series": dict(single["u32_after_clock_series"], frac=round(
                     single["u32"]["bytes"] / single["u32_after_clock_series"]["last15_kernel_ms"] / 1e6 / PEAK_GBS,
                     4))},
+        "f64_matrix_core": {
+            "kernel": "k_fill_mfma<F64,64,16>: the matrix-core fill for doubles (outputs_per_lane 16, "
+                      "1024-thread blocks, 2048 per wave), measured after the word legs",
+            "kernel_ms": round(single["f64_matrix_core"]["kernel_ms"], 4),
+            "frac": round(single["f64_matrix_core"]["bytes"] / (single["f64_matrix_core"]["kernel_ms"] * 1e-3) / 1e9
+                          / PEAK_GBS, 4),
+            "per_launch_ms": [round(x, 4) for x in single["f64_matrix_core"]["per_launch_ms"]]},
         "small_2e24_f64": {k: round(v, 4) for k, v in single["small"].items()},
         "write_ceiling": {"kernel": "mrg8_probe_write: d[i] = i over the same 2 GiB with the fill's 1 KiB "
                                     "non-temporal wave stores (the counterpart of cuda/stream.cu), measured "

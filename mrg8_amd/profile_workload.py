@@ -8,8 +8,13 @@ Dispatches, in order (each `--warmup` + `--reps` times; summaries use the
 last `--reps` dispatches of each kind):
   1. a runtime device-to-device copy of 1 GiB  -- the calibration kernel of
      known traffic (1 GiB read + 1 GiB written)
-  2. single-stream fill, 2^28 float64 outputs (seed 13579)
-  3. single-stream fill, 2^28 uint32 outputs
+  2. single-stream fill, 2^28 float64 outputs (seed 13579), default shape
+     (the vector kernel)
+  3. single-stream fill, 2^28 uint32 outputs, default shape (the
+     matrix-core fill)
+  3a. the same word fill with the vector kernel, and the float64 fill with
+     the matrix-core fill (outputs_per_lane 16, 1024-thread blocks, 2048 per
+     wave)
   4. batched launch: 4096 streams, log-uniform lengths, float64, one
      2048-output tile per wave, 1024-thread blocks                (--batched)
   5. jump-heavy batched launch: 65536 x (63-bit jump + 256 uint32) (--batched)
@@ -61,9 +66,10 @@ def main():
     dst.free()
     buf = hip.DeviceBuffer(n * 8)
     g = mrg8_amd.mrg8_cuda(workloads.SEED)
-    for fmt in (mrg8_amd.F64, mrg8_amd.U32):
+    for fmt, shape in ((mrg8_amd.F64, {}), (mrg8_amd.U32, {}), (mrg8_amd.U32, {"per_lane": 4}),
+                       (mrg8_amd.F64, {"per_lane": 16, "block_threads": 1024, "tile": 2048})):
         for _ in range(args.warmup + args.reps):
-            g.generate(buf, n, fmt)
+            g.generate(buf, n, fmt, **shape)
         g.synchronize()
     buf.free()
     if args.batched:
