@@ -123,3 +123,32 @@ def test_shape_errors(gpu):
                 g.generate(buf.ptr, 100_000, 2, **kw)
     finally:
         buf.free()
+
+
+@pytest.mark.parametrize("fmt,per_lane", [(2, 0), (0, 16)])
+def test_inside_a_hip_graph(gpu, oracle_mod, fmt, per_lane):
+    """The matrix-core fill captured into a HIP graph (its tables live in the
+    device context, so the capture records one kernel node and no
+    allocation): every replay rewrites the outputs of the positions the
+    handle had at capture, identical to the oracle."""
+    import torch
+
+    n = (1 << 22) + 333
+    dt = torch.int32 if fmt == 2 else torch.float64
+    g = gpu.mrg8_cuda(SEED)
+    g.jump_ahead(77)
+    out = torch.zeros(n, dtype=dt, device="cuda:0")
+    side = torch.cuda.Stream()
+    graph = torch.cuda.CUDAGraph()
+    say("mfma-graph", fmt, per_lane)
+    with torch.cuda.graph(graph, stream=side):
+        g.generate(out, n, fmt, stream=side, per_lane=per_lane)
+    s0 = oracle_mod.jump(oracle_mod.seed_state(SEED), 77)
+    want, st = oracle_mod.generate(s0, n, fmt)
+    assert np.array_equal(g.get_state(), st)
+    for rep in range(3):
+        out.zero_()
+        torch.cuda.synchronize()
+        graph.replay()
+        torch.cuda.synchronize()
+        assert out.cpu().numpy().tobytes() == want.tobytes(), rep
