@@ -25,8 +25,8 @@ With --clock (no profiler needed) it instead prints, per launch, the kernel
 time (HIP events) and the shader clock sampled right after it, for a
 sequence started after 2 s idle: six 2^28 float64 fills, then twelve rounds
 of { fill, write probe with the fill's store pattern, hipMemsetAsync, 2^28
-uint32 fill } over the same 2 GiB -- which kernels slow down while the power
-manager holds the clock down.
+uint32 fill with the vector kernel } over the same 2 GiB -- which kernels
+slow down while the power manager holds the clock down.
 """
 from __future__ import annotations
 
@@ -132,7 +132,7 @@ def clock_interleave():
     seq += [("fill f64", lambda: g.generate(buf, n, mrg8_amd.F64, stream=s)),
             ("write probe", lambda: probe.write(buf.ptr, n, 1, 0, s)),
             ("memset", lambda: hip.memset_async(buf.ptr, 0, n * 8, s)),
-            ("fill u32", lambda: g.generate(buf, n, mrg8_amd.U32, stream=s))] * 12
+            ("fill u32", lambda: g.generate(buf, n, mrg8_amd.U32, stream=s, per_lane=4))] * 12
     clk = probe.ClockProbe(len(seq))
     ev = [(hip.Event(), hip.Event()) for _ in seq]
     s.synchronize()
