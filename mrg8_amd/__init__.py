@@ -107,7 +107,15 @@ class mrg8_cuda:
     # -- device generation ------------------------------------------------------
     def generate(self, dptr, n: int, fmt: int = F64, stream=None, block_threads: int = 0, tile: int = 0,
                  per_lane: int = 0) -> None:
-        """Stream-ordered fill of n outputs at device address `dptr`; the state advances by n."""
+        """Stream-ordered fill of n outputs at device address `dptr`; the state advances by n.
+
+        Shape (0 = automatic, the output is the same for every shape):
+        per_lane 1, 2 or 4 selects the vector kernel (block_threads a multiple
+        of 64 up to 1024, tile = outputs per wave); 16 or 32 the matrix-core
+        fill (int8 MFMAs over byte digits; tiles of 16 columns of 4 * per_lane
+        outputs, block_threads 256, 512 or 1024, tile = outputs per wave, a
+        multiple of the tile).  Automatic: the matrix-core fill for word fills
+        of 2^22 outputs or more, the vector kernel otherwise."""
         if block_threads or tile or per_lane:
             capi.call("mrg8_generate_shape", self._h, ctypes.c_void_p(_ptr(dptr)), ctypes.c_uint64(n), fmt,
                       _stream(stream), block_threads, ctypes.c_uint64(tile), per_lane)
