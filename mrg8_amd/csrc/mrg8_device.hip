@@ -729,9 +729,9 @@ __global__ __launch_bounds__(256) void k_stage(const u4_t* __restrict__ src, u4_
 //     2^29, and x = t + ((u << 16) & p) + (u >> 15) (2^16 u mod p is a 31-bit
 //     rotation) is the residue plus at most one p: eight 32-bit operations
 //     per output.
-// MFMA layouts (gfx950, checked on the device): lane l = n + 16 g holds the
-// 8 bytes of K-group g for row/column n, and result register i of lane l is
-// row 4 g + i of column n.  K-group g, byte i is word W(g) + i / 4, digit
+// MFMA layouts (gfx950; the fill's parity tests depend on them): lane
+// l = n + 16 g holds the 8 bytes of K-group g for row/column n, and result
+// register i of lane l is row 4 g + i of column n.  K-group g, byte i is word W(g) + i / 4, digit
 // i % 4, with W = (0, 4, 2, 6); row 4 g + i is step position 2 g + i for
 // i < 2 and 8 + W(g) + i - 2 for i >= 2.  So the step's last eight outputs
 // (positions 8..15, the column's next state) are registers 2 and 3 of
