@@ -10,7 +10,8 @@ benchmark configurations.  Printed value: G samples/s over the timed steps.
 The same JSON line carries, measured in the same run (in this order):
   write_ceiling the project's write probe (d[i] = i, the counterpart of the
                 reference's cuda/stream.cu) over the same 2 GiB with the fill's
-                store pattern and in the reference's form, and hipMemsetAsync
+                store pattern (and its runs in dispatch order) and in the
+                reference's form, and hipMemsetAsync
   roofline      algorithmic bytes per launch / mean kernel time (HIP events on
                 the launch stream, one between every two launches; every
                 launch's duration listed) against 8 TB/s; HBM traffic from the
@@ -133,14 +134,17 @@ def timed_series(step, stream, warmup, steps):
 
 def write_ceiling(buf, nbytes, s, reps=10):
     """Store-bandwidth ceiling over the same buffer: this project's probe
-    (d[i] = i, the counterpart of cuda/stream.cu) with the fill's 1 KiB
-    non-temporal wave stores and in the reference's one-element-per-thread
-    form, and the runtime's own fill kernel (hipMemsetAsync)."""
+    (d[i] = i, the counterpart of cuda/stream.cu) with the double fill's 1 KiB
+    non-temporal wave stores (512-thread blocks in XCD runs), with the same
+    runs in 1024-thread blocks in dispatch order, and in the reference's
+    one-element-per-thread form, and the runtime's own fill kernel
+    (hipMemsetAsync)."""
     from mrg8_amd import hip, probe
 
     n = nbytes // 8
     out = {}
     for name, fn in (("probe_fill_pattern", lambda: probe.write(buf.ptr, n, 1, 0, s)),
+                     ("probe_dispatch_order", lambda: probe.write(buf.ptr, n, 2, 0, s)),
                      ("probe_reference_form", lambda: probe.write(buf.ptr, n, 0, 0, s)),
                      ("memset", lambda: hip.memset_async(buf.ptr, 0, nbytes, s))):
         fn()
@@ -595,10 +599,11 @@ def main():
             "per_launch_ms": [round(x, 4) for x in single["f64_matrix_core"]["per_launch_ms"]]},
         "small_2e24_f64": {k: round(v, 4) for k, v in single["small"].items()},
         "write_ceiling": {"kernel": "mrg8_probe_write: d[i] = i over the same 2 GiB with the fill's 1 KiB "
-                                    "non-temporal wave stores (the counterpart of cuda/stream.cu), measured "
-                                    "just before the float64 leg",
+                                    "non-temporal wave stores, 512-thread blocks in XCD runs of 64 (the "
+                                    "counterpart of cuda/stream.cu), measured just before the float64 leg",
                           "gbs": ceiling["gbs"], "frac": round(ceiling["gbs"] / PEAK_GBS, 4),
                           "fill_frac_of_ceiling": round(achieved / ceiling["gbs"], 4),
+                          "dispatch_order_gbs": single["ceiling"]["probe_dispatch_order"]["gbs"],
                           "reference_form_gbs": single["ceiling"]["probe_reference_form"]["gbs"],
                           "memset_gbs": single["ceiling"]["memset"]["gbs"]},
         "clock": dict(single["clock"], note=(

@@ -14,9 +14,11 @@ extern "C" {
 
 /* d[i] = (double)i for i < n: the write-bandwidth calibration of the
  * reference (cuda/stream.cu:8-15, one element per thread in 512-thread
- * blocks) with mode 0; with mode 1 the generator fill's store pattern
- * (runs of `tile` doubles per wave, 0 = 2048, a multiple of 128; 1 KiB
- * non-temporal wave stores; d 1 KiB-aligned) and no arithmetic. */
+ * blocks) with mode 0; with mode 1 the store pattern of the generator's
+ * double fill (runs of `tile` doubles per wave, 0 = 2048, a multiple of 128;
+ * 1 KiB non-temporal wave stores; d 1 KiB-aligned; 512-thread blocks, each
+ * XCD taking runs of 64 consecutive blocks' output) and no arithmetic; mode 2
+ * the same runs in 1024-thread blocks in dispatch order. */
 int mrg8_probe_write(void* d, uint64_t n, int mode, uint64_t tile, void* stream);
 
 /* `blocks` blocks of 256 threads, each thread running eight independent

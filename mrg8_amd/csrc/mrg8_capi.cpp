@@ -118,6 +118,13 @@ int device_context(int device, DeviceContext** out) {
   auto ctx = std::make_unique<DeviceContext>();
   ctx->device = device;
   ctx->num_cu = prop.multiProcessorCount;
+  int xcc = 1;
+  if (hipDeviceGetAttribute(&xcc, hipDeviceAttributeNumberOfXccs, device) != hipSuccess) {
+    (void)hipGetLastError();  // not reported by a later launch check
+    xcc = 1;
+  }
+  if (xcc < 1) xcc = 1;
+  ctx->num_xcc = xcc;
   void* p = nullptr;
   const size_t rec_bytes = sizeof(CoefRecord) * (lane.size() + jump.size());
   // operand digits and accumulator start values do not depend on the column length
@@ -229,10 +236,10 @@ int format_size(int format) {
 }
 
 // Default launch shape for a single-stream fill of n outputs.  Measured on
-// MI355X (2^28 outputs): doubles are fastest with 1024-thread blocks, 4
-// outputs per lane and 2048 outputs per wave, words with 256-thread blocks
-// and 8192 per wave; small fills shrink the per-wave run (down to four
-// iterations) so that every SIMD still gets waves.
+// MI355X (2^28 outputs): doubles are fastest with 512-thread blocks in XCD
+// runs (fill_xcd_group), 4 outputs per lane and 2048 outputs per wave, words
+// with 256-thread blocks and 8192 per wave; small fills shrink the per-wave
+// run (down to four iterations) so that every SIMD still gets waves.
 void default_shape(const DeviceContext& ctx, uint64_t n, int format, int per_lane, int* block, uint64_t* tile) {
   const uint64_t quantum = 64ull * per_lane;
   uint64_t t = format == MRG8_U32 ? 8192 : 2048;
@@ -240,14 +247,29 @@ void default_shape(const DeviceContext& ctx, uint64_t n, int format, int per_lan
   const uint64_t min_waves = cu * 4 * 4;  // four waves per SIMD
   while (t > 4 * quantum && (n + t - 1) / t < min_waves) t /= 2;
   *tile = std::max<uint64_t>(quantum, t / quantum * quantum);
-  // Double fills take 1024-thread blocks once every CU gets 128 waves or
-  // more; smaller fills spread over more, smaller blocks (on MI355X, against
-  // 1024-thread blocks: 2^20 and 2^21 doubles 15-44 % faster, 2^22..2^25
-  // 0-4 % faster, 2^26 2-3 % slower in 512-thread blocks).
-  int b = format == MRG8_U32 ? 256 : 1024;
+  // Double fills take 512-thread blocks, and 256-thread blocks when every CU
+  // gets at most 4 waves (on MI355X, against 1024-thread blocks: 2^20 and
+  // 2^21 doubles 15-44 % faster, 2^22..2^25 0-4 % faster; from 2^25 on in
+  // XCD runs, 2^26 the same, 2^27 1 % and 2^28 1-2 % faster at full clock
+  // and 3.5-5 % faster in the first launches after an idle spell, while the
+  // power manager holds the clock down).
+  int b = format == MRG8_U32 ? 256 : 512;
   const uint64_t waves = (n + *tile - 1) / *tile;
-  if (format != MRG8_U32 && waves < cu * 128) b = waves <= cu * 4 ? 256 : 512;
+  if (format != MRG8_U32 && waves <= cu * 4) b = 256;
   *block = b;
+}
+
+// Blocks per XCD run of a vector-kernel fill (1: blocks in dispatch order).
+// The dispatcher deals blocks to the XCDs in turn; a double fill of 64 or
+// more waves per CU instead gives each XCD runs of 32768 threads' worth of
+// consecutive output (64 blocks of 512 threads).  Measured on MI355X, 2^28
+// doubles at 2048 per wave: 512-thread blocks 366 us in dispatch order and
+// 339-341 us in runs of 64; 256-thread blocks 391 and 340 us (runs of 128);
+// 1024-thread blocks 338-349 us either way.  Word fills (8192 per wave) were
+// 1-2 % slower in runs, and are left in dispatch order.
+uint32_t fill_xcd_group(const DeviceContext& ctx, int format, uint64_t waves, int block_threads) {
+  if (format == MRG8_U32 || ctx.num_xcc <= 1 || waves < (uint64_t)ctx.num_cu * 64) return 1;
+  return std::max(1, 32768 / block_threads);
 }
 
 namespace {
@@ -382,6 +404,8 @@ int fill_device(DeviceContext& ctx, const uint32_t state[8], uint64_t offset, vo
   a.start.offset = offset;
   a.lane = ctx.tables.lane;
   a.jump = ctx.tables.jump;
+  a.xcds = (uint32_t)ctx.num_xcc;
+  a.xcd_group = fill_xcd_group(ctx, format, fill_waves(n, head, tile), block_threads);
   int prev = 0;
   (void)hipGetDevice(&prev);
   if (prev != ctx.device) (void)hipSetDevice(ctx.device);

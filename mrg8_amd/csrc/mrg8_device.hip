@@ -416,6 +416,20 @@ __device__ __forceinline__ void generate_run(typename Out<FMT>::T* __restrict__ 
 // before the first tile-aligned address), every later wave starts on one.
 __device__ __forceinline__ uint64_t fill_first(const FillArgs& a, uint64_t w) { return w ? a.head + w * a.tile : 0; }
 
+// The part of the output that block b of an nb-block grid writes.  The
+// dispatcher deals blocks to the XCDs in turn (block b to XCD b mod x), so in
+// dispatch order the XCDs interleave block by block.  With group g > 1, each
+// XCD instead takes runs of g consecutive blocks' output within every span of
+// x g blocks.  Blocks past the last whole span keep their own part.  Either
+// way every part is written by exactly one block.
+__device__ __forceinline__ uint64_t xcd_block(uint32_t b, uint32_t nb, uint32_t x, uint32_t g) {
+  if (g <= 1 || x == 0) return b;
+  const uint32_t span = x * g, whole = nb / span * span;
+  if (b >= whole) return b;
+  const uint32_t q = b / x;
+  return (uint64_t)(q / g * span + (b - q * x) * g + q % g);
+}
+
 template <int FMT, int PL>
 __global__ __launch_bounds__(1024) void k_fill(FillArgs a) {
   // Start states of the block's waves: wave j (j < ceil(nw / 8)) jumps the
@@ -425,7 +439,7 @@ __global__ __launch_bounds__(1024) void k_fill(FillArgs a) {
   const int lane = threadIdx.x & 63;
   const uint32_t wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform wave index
   const uint32_t nw = blockDim.x >> 6;
-  const uint64_t wave0 = (uint64_t)blockIdx.x * nw;
+  const uint64_t wave0 = xcd_block(blockIdx.x, gridDim.x, a.xcds, a.xcd_group) * nw;
   if (wib * 8 < nw) {
     const uint32_t target = wib * 8 + (lane >> 3);
     uint32_t s[8];

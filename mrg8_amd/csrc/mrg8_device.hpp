@@ -84,6 +84,8 @@ struct FillArgs {
   StreamStart start;        // the state, and the position (offset) of out[0]
   const CoefRecord* lane;
   const CoefRecord* jump;
+  uint32_t xcds;            // XCDs that blocks are dealt to in turn (block b runs on XCD b mod xcds)
+  uint32_t xcd_group;       // consecutive blocks of output per XCD run (1: blocks in dispatch order)
 };
 
 // A matrix-core fill: `tiles` whole tiles of 16 x `column` outputs at

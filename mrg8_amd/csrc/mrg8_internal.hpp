@@ -23,6 +23,7 @@ struct TunedShape {
 struct DeviceContext {
   int device = 0;
   int num_cu = 0;
+  int num_xcc = 1;  // XCDs the dispatcher deals blocks to in turn
   DeviceTables tables;
   TunedShape tuned[3][64];  // [format][floor(log2 n)], set by mrg8_tune_fill
   std::mutex tuned_mu;      // guards `tuned` (fills on other handles read it)

@@ -3,9 +3,11 @@
 //  * Write probe: d_a[i] = i over n doubles, the counterpart of the
 //    reference's bandwidth calibration (cuda/stream.cu:8-15,42-56).  Mode 0
 //    keeps its shape (one element per thread, 512-thread blocks, plain
-//    stores); mode 1 writes with the store pattern of the generator's fill
-//    kernel and no arithmetic (each wave one run of `tile` doubles, 16 bytes
-//    per lane and 1 KiB per wave store, non-temporal).
+//    stores); mode 1 writes with the store pattern of the generator's double
+//    fill and no arithmetic (each wave one run of `tile` doubles, 16 bytes
+//    per lane and 1 KiB per wave store, non-temporal; 512-thread blocks, each
+//    XCD taking runs of 64 consecutive blocks' output); mode 2 the same runs
+//    in 1024-thread blocks in dispatch order.
 //  * 64-bit multiply-add peak: every lane runs eight independent chains of
 //    v_mad_i64_i32 (the generator's multiply-add), the VALU-side ceiling of
 //    its integer work.
@@ -27,12 +29,25 @@ __global__ __launch_bounds__(512) void k_write_elem(double* __restrict__ d, uint
   if (i < n) d[i] = (double)i;
 }
 
+// Block b of nb writes the output of block xcd_block(b): blocks are dealt to
+// the x XCDs in turn, and with g > 1 each XCD takes runs of g consecutive
+// blocks' output within every span of x g blocks (blocks past the last whole
+// span keep their own).  The generator's fill maps its blocks the same way.
+__device__ __forceinline__ uint64_t xcd_block(uint32_t b, uint32_t nb, uint32_t x, uint32_t g) {
+  if (g <= 1 || x == 0) return b;
+  const uint32_t span = x * g, whole = nb / span * span;
+  if (b >= whole) return b;
+  const uint32_t q = b / x;
+  return (uint64_t)(q / g * span + (b - q * x) * g + q % g);
+}
+
 // Wave w writes elements [w * tile, min((w + 1) * tile, n)); tile is a
 // multiple of 128 and d is 1 KiB-aligned (checked by the caller), so every
 // full wave store covers 1 KiB of whole cache lines.
-__global__ __launch_bounds__(1024) void k_write_tiles(double* __restrict__ d, uint64_t n, uint64_t tile) {
+__global__ __launch_bounds__(1024) void k_write_tiles(double* __restrict__ d, uint64_t n, uint64_t tile, uint32_t x,
+                                                      uint32_t g) {
   const uint32_t lane = threadIdx.x & 63;
-  const uint64_t w = (uint64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const uint64_t w = xcd_block(blockIdx.x, gridDim.x, x, g) * (blockDim.x >> 6) + (threadIdx.x >> 6);
   const uint64_t first = w * tile;
   if (first >= n) return;
   const uint64_t end = first + tile < n ? first + tile : n;
@@ -89,13 +104,25 @@ extern "C" int mrg8_probe_write(void* d, uint64_t n, int mode, uint64_t tile, vo
     const uint64_t blocks = (n + 511) / 512;
     if (blocks > 0xffffffffull) return -1;
     hipLaunchKernelGGL(k_write_elem, dim3((uint32_t)blocks), dim3(512), 0, s, static_cast<double*>(d), n);
-  } else if (mode == 1) {
+  } else if (mode == 1 || mode == 2) {
     if (tile == 0) tile = 2048;
     if (tile % 128 || (reinterpret_cast<uintptr_t>(d) & 1023)) return -1;
+    const uint32_t threads = mode == 1 ? 512 : 1024;
+    uint32_t xcds = 1, group = 1;
+    if (mode == 1) {
+      int dev = 0, x = 1;
+      if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&x, hipDeviceAttributeNumberOfXccs, dev) != hipSuccess) {
+        (void)hipGetLastError();
+        x = 1;
+      }
+      xcds = x > 1 ? (uint32_t)x : 1;
+      group = xcds > 1 ? 32768 / threads : 1;
+    }
     const uint64_t waves = (n + tile - 1) / tile;
-    const uint64_t blocks = (waves + 15) / 16;
+    const uint64_t blocks = (waves + threads / 64 - 1) / (threads / 64);
     if (blocks > 0xffffffffull) return -1;
-    hipLaunchKernelGGL(k_write_tiles, dim3((uint32_t)blocks), dim3(1024), 0, s, static_cast<double*>(d), n, tile);
+    hipLaunchKernelGGL(k_write_tiles, dim3((uint32_t)blocks), dim3(threads), 0, s, static_cast<double*>(d), n, tile,
+                       xcds, group);
   } else {
     return -1;
   }

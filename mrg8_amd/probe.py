@@ -33,7 +33,9 @@ def _s(stream):
 
 def write(dptr: int, n: int, mode: int = 1, tile: int = 0, stream=None) -> None:
     """d[i] = i for n doubles at device address dptr (mode 0: one element per
-    thread as cuda/stream.cu; mode 1: the fill's 1 KiB non-temporal wave stores)."""
+    thread as cuda/stream.cu; mode 1: the double fill's 1 KiB non-temporal
+    wave stores, 512-thread blocks in XCD runs; mode 2: the same runs in
+    1024-thread blocks in dispatch order)."""
     rc = lib().mrg8_probe_write(ctypes.c_void_p(int(dptr)), n, mode, tile, _s(stream))
     if rc != 0:
         raise RuntimeError(f"mrg8_probe_write failed ({rc})")

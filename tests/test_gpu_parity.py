@@ -180,6 +180,36 @@ def test_launch_shapes_identical(gpu, oracle_mod, block, tile):
     assert np.array_equal(got.view(np.uint64), want.view(np.uint64))
 
 
+@pytest.mark.parametrize("block", [64, 256, 512, 1024])
+@pytest.mark.parametrize("fmt,pad", [(0, 1), (1, 0)])
+def test_xcd_runs_cover_every_wave(gpu, oracle_mod, block, fmt, pad):
+    """Double fills of 64 or more waves per CU give each XCD runs of
+    consecutive blocks' output.  With 256-output waves the grid has whole
+    spans of runs plus a remainder of blocks in dispatch order; the buffer is
+    poisoned first, so a part written twice or never shows up."""
+    import torch
+    from mrg8_amd.hip import DeviceBuffer
+
+    cu = torch.cuda.get_device_properties(0).multi_processor_count
+    n = cu * 64 * 256 * 19 // 16 + 3
+    say("xcd-runs", block, fmt, pad, n)
+    buf = DeviceBuffer((n + pad) * 8)
+    try:
+        buf.fill_byte(0xA5)
+        g = gpu.mrg8_cuda(SEED)
+        g.jump_ahead(4242)
+        g.generate(buf.ptr + pad * 8, n, fmt, block_threads=block, tile=256)
+        g.synchronize()
+        got = buf.copy_to_host(np.float64, n, pad * 8)
+        s0 = oracle_mod.jump(oracle_mod.seed_state(SEED), 4242)
+        want, st = oracle_mod.generate(s0, n, fmt)
+        bad = np.flatnonzero(got.view(np.uint64) != want.view(np.uint64))
+        assert bad.size == 0, f"{bad.size} outputs differ, first at {bad[:4]}"
+        assert np.array_equal(g.get_state(), st)
+    finally:
+        buf.free()
+
+
 def test_beyond_2e32_outputs(gpu, oracle_mod):
     """One call of 2^32 + 4096 words (16 GiB): 64-bit counts and offsets
     throughout (the reference takes an int n and jumps 32 bits on the
