@@ -269,7 +269,7 @@ void default_shape(const DeviceContext& ctx, uint64_t n, int format, int per_lan
 // 1-2 % slower in runs, and are left in dispatch order.
 uint32_t fill_xcd_group(const DeviceContext& ctx, int format, uint64_t waves, int block_threads) {
   if (format == MRG8_U32 || ctx.num_xcc <= 1 || waves < (uint64_t)ctx.num_cu * 64) return 1;
-  return std::max(1, 32768 / block_threads);
+  return xcd_run_blocks(ctx, block_threads);
 }
 
 namespace {

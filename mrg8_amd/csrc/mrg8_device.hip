@@ -22,6 +22,9 @@
 //     jumped by its first waves and shared through LDS.
 //   * k_batch: many streams (segments), a persistent grid striding over
 //     tiles, tables and counts read from one device block.
+//   * Blocks reach their part of the output through xcd_block: the
+//     dispatcher deals blocks to the XCDs in turn, and large grids give each
+//     XCD runs of consecutive blocks' output instead.
 #include "mrg8_device.hpp"
 
 namespace mrg8 {
@@ -538,7 +541,8 @@ __global__ __launch_bounds__(1024) void k_batch(BatchArgs a) {
   const BatchTables tab = batch_tables(a.block);
   const uint64_t ntiles = tab.ntiles;
   const uint64_t nwaves = (uint64_t)gridDim.x * (blockDim.x >> 6);
-  const uint64_t wave = (uint64_t)blockIdx.x * (blockDim.x >> 6) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const uint64_t wave =
+      xcd_block(blockIdx.x, gridDim.x, a.xcds, a.xcd_group) * (blockDim.x >> 6) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   if (wave >= ntiles) return;
   const uint64_t mine_total = (ntiles - wave + nwaves - 1) / nwaves;
   for (uint64_t j0 = 0; j0 < mine_total; j0 += kGroup) {
@@ -588,7 +592,8 @@ __global__ __launch_bounds__(1024) void k_batch_tiles(BatchArgs a) {
   const BatchTables tab = batch_tables(a.block);
   const uint32_t wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const uint32_t nw = blockDim.x >> 6;
-  const uint64_t tile0 = (uint64_t)blockIdx.x * nw;  // < ntiles: the grid covers the tiles exactly
+  // < ntiles: the grid covers the tiles exactly
+  const uint64_t tile0 = xcd_block(blockIdx.x, gridDim.x, a.xcds, a.xcd_group) * nw;
   if (wib * 8 < nw) {
     const uint32_t target = wib * 8 + (lane >> 3);
     const bool mine = target < nw && tile0 + target < tab.ntiles;

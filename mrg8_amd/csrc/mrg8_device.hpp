@@ -195,6 +195,8 @@ struct BatchArgs {
   const uint32_t* states;      // resident per-stream states (8 words each), may be null
   const CoefRecord* lane;
   const CoefRecord* jump;
+  uint32_t xcds;               // as in FillArgs
+  uint32_t xcd_group;
 };
 
 // A bulk advance: out[8i..8i+7] = in[8i..8i+7] jumped ahead by the

@@ -31,6 +31,12 @@ struct DeviceContext {
   uint8_t* noalloc_scratch = nullptr;  // this device's copy of the fixed scratch area
 };
 
+// Blocks per XCD run for kernels that map blocks through xcd_block: runs of
+// 32768 threads (64 blocks of 512) on a device with more than one XCD.
+inline uint32_t xcd_run_blocks(const DeviceContext& ctx, int block_threads) {
+  return ctx.num_xcc > 1 && block_threads > 0 ? (uint32_t)std::max(1, 32768 / block_threads) : 1;
+}
+
 // Builds the table of byte positions kJumpPositions..kWidePositions-1 on the
 // context's device if it is not there yet.
 int ensure_wide_table(DeviceContext& ctx);

@@ -112,6 +112,8 @@ BatchArgs BatchSlot::args(const LaunchConfig& cfg) const {
   a.states = states_;
   a.lane = ctx_->tables.lane;
   a.jump = ctx_->tables.jump;
+  a.xcds = (uint32_t)ctx_->num_xcc;
+  a.xcd_group = xcd_run_blocks(*ctx_, cfg.block_threads);
   return a;
 }
 
