@@ -9,14 +9,15 @@ last `--reps` dispatches of each kind):
   1. a runtime device-to-device copy of 1 GiB  -- the calibration kernel of
      known traffic (1 GiB read + 1 GiB written)
   2. single-stream fill, 2^28 float64 outputs (seed 13579), default shape
-     (the vector kernel)
+     (the vector kernel: 512-thread blocks in XCD runs of 64)
   3. single-stream fill, 2^28 uint32 outputs, default shape (the
      matrix-core fill)
   3a. the same word fill with the vector kernel, and the float64 fill with
      the matrix-core fill (outputs_per_lane 16, 1024-thread blocks, 2048 per
      wave)
   4. batched launch: 4096 streams, log-uniform lengths, float64, one
-     2048-output tile per wave, 1024-thread blocks                (--batched)
+     2048-output tile per wave, 1024-thread blocks in XCD runs of 32
+                                                                   (--batched)
   5. jump-heavy batched launch: 65536 x (63-bit jump + 256 uint32) (--batched)
   6. bulk advance of 2^20 device-resident states by random 200-bit offsets,
      then by random 64-bit offsets                                 (--advance)
