@@ -264,9 +264,9 @@ void default_shape(const DeviceContext& ctx, uint64_t n, int format, int per_lan
 // more waves per CU instead gives each XCD runs of 32768 threads' worth of
 // consecutive output (64 blocks of 512 threads).  Measured on MI355X, 2^28
 // doubles at 2048 per wave: 512-thread blocks 366 us in dispatch order and
-// 339-341 us in runs of 64; 256-thread blocks 391 and 340 us (runs of 128);
+// 337-341 us in runs of 64; 256-thread blocks 391 and 340 us (runs of 128);
 // 1024-thread blocks 338-349 us either way.  Word fills (8192 per wave) were
-// 1-2 % slower in runs, and are left in dispatch order.
+// 1-17 % slower in runs, and are left in dispatch order.
 uint32_t fill_xcd_group(const DeviceContext& ctx, int format, uint64_t waves, int block_threads) {
   if (format == MRG8_U32 || ctx.num_xcc <= 1 || waves < (uint64_t)ctx.num_cu * 64) return 1;
   return xcd_run_blocks(ctx, block_threads);
