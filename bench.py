@@ -574,7 +574,10 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_GBS, 4),
                      "traffic": traffic,
-                     "kernel": "k_fill<F64,4>", "kernel_ms": round(f64["kernel_ms"], 4),
+                     "kernel": "k_fill<F64,4>",
+                     "shape": "default: 512-thread blocks, 2048 outputs per wave, 4 per lane, blocks in XCD runs "
+                              "of 64 (each XCD writes 64 consecutive blocks' output)",
+                     "kernel_ms": round(f64["kernel_ms"], 4),
                      "per_launch_ms": [round(x, 4) for x in f64["per_launch_ms"]],
                      "valu_busy_pct": valu_busy_from_profiles("fill_f64_2e28_counters.txt", "k_fill<F64,4>"),
                      "algorithmic_bytes_per_launch": f64["bytes"]},
