@@ -33,6 +33,12 @@ int mrg8_probe_mad64(uint64_t* d_out, uint32_t blocks, int iters, void* stream);
  * the same interval: clock in MHz = 100 * d_out[0] / d_out[1]. */
 int mrg8_probe_sclk(uint64_t* d_out, void* stream);
 
+/* `blocks` blocks of one wave each; block b writes d_out[b] = the XCD it ran
+ * on (bits [3:0] of the XCC_ID hardware register).  The fills and the
+ * batched kernels map their blocks to the output in XCD runs on the
+ * assumption that block b runs on XCD b mod (number of XCDs). */
+int mrg8_probe_xcc(uint32_t* d_out, uint32_t blocks, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -14,6 +14,9 @@
 //  * Shader-clock probe: one wave counts s_memtime (shader clock) ticks
 //    against the constant 100 MHz s_memrealtime over about 20 us, so a
 //    caller can see the clock the power manager has set between launches.
+//  * XCD probe: each block of a grid records the XCD (XCC_ID hardware
+//    register) it ran on, which shows how the dispatcher deals blocks to the
+//    XCDs -- the assumption behind the fills' XCD runs.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -94,6 +97,12 @@ __global__ void k_sclk(uint64_t* __restrict__ out) {
   }
 }
 
+// XCC_ID (hardware register 20 on gfx9.4/9.5): bits [3:0] are the XCD.
+__global__ void k_xcc(uint32_t* __restrict__ out) {
+  const uint32_t id = __builtin_amdgcn_s_getreg(20 | (0 << 6) | (3 << 11));
+  if (threadIdx.x == 0) out[blockIdx.x] = id;
+}
+
 }  // namespace
 
 extern "C" int mrg8_probe_write(void* d, uint64_t n, int mode, uint64_t tile, void* stream) {
@@ -132,6 +141,12 @@ extern "C" int mrg8_probe_write(void* d, uint64_t n, int mode, uint64_t tile, vo
 extern "C" int mrg8_probe_mad64(uint64_t* d_out, uint32_t blocks, int iters, void* stream) {
   if (!d_out || blocks == 0 || iters <= 0) return -1;
   hipLaunchKernelGGL(k_mad64, dim3(blocks), dim3(256), 0, static_cast<hipStream_t>(stream), d_out, iters);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+extern "C" int mrg8_probe_xcc(uint32_t* d_out, uint32_t blocks, void* stream) {
+  if (!d_out || blocks == 0) return -1;
+  hipLaunchKernelGGL(k_xcc, dim3(blocks), dim3(64), 0, static_cast<hipStream_t>(stream), d_out);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 

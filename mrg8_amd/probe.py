@@ -23,12 +23,28 @@ def lib() -> ctypes.CDLL:
                                        ctypes.c_void_p]
         L.mrg8_probe_sclk.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
         L.mrg8_probe_mad64.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int, ctypes.c_void_p]
+        L.mrg8_probe_xcc.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p]
         _lib = L
     return _lib
 
 
 def _s(stream):
     return ctypes.c_void_p(stream.handle if hasattr(stream, "handle") else stream)
+
+
+def xcc_ids(blocks: int = 4096) -> np.ndarray:
+    """The XCD each of `blocks` one-wave blocks ran on (block order)."""
+    from . import hip
+
+    buf = hip.DeviceBuffer(4 * blocks)
+    try:
+        rc = lib().mrg8_probe_xcc(ctypes.c_void_p(buf.ptr), blocks, None)
+        if rc != 0:
+            raise RuntimeError(f"mrg8_probe_xcc failed ({rc})")
+        hip.synchronize()
+        return buf.copy_to_host(np.uint32, blocks)
+    finally:
+        buf.free()
 
 
 def write(dptr: int, n: int, mode: int = 1, tile: int = 0, stream=None) -> None:

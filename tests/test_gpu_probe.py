@@ -1,6 +1,7 @@
-"""The benchmark's write probe (libmrg8_probe.so) writes d[i] = i over the
-whole buffer in each of its store patterns, including the XCD runs and the
-blocks past the last whole span of runs."""
+"""The benchmark's probes (libmrg8_probe.so): the write probe writes
+d[i] = i over the whole buffer in each of its store patterns, including the
+XCD runs and the blocks past the last whole span of runs; the dispatcher
+deals blocks to the XCDs in turn, the assumption behind the XCD runs."""
 import numpy as np
 import pytest
 
@@ -22,3 +23,16 @@ def test_write_probe_covers_buffer(gpu, mode, tile):
         assert np.array_equal(got, np.arange(n, dtype=np.float64))
     finally:
         buf.free()
+
+
+def test_blocks_dealt_to_xcds_in_turn(gpu):
+    """Block b runs on XCD b mod X, X the number of XCDs the grid reaches.
+    The fills' outputs do not depend on it (xcd_block is a permutation of the
+    blocks either way); their speed does."""
+    from mrg8_amd import probe
+
+    print("[case] xcc ids", flush=True)
+    ids = probe.xcc_ids(4096)
+    x = len(np.unique(ids))
+    assert x >= 1 and set(np.unique(ids)) == set(range(x))
+    assert np.array_equal(ids, np.arange(4096, dtype=np.uint32) % x), ids[:24]

@@ -49,7 +49,7 @@ def test_probe_library_exports_its_header():
     from mrg8_amd import probe
 
     names = header_functions(only="mrg8_probe.h")
-    assert names == {"mrg8_probe_write", "mrg8_probe_sclk", "mrg8_probe_mad64"}
+    assert names == {"mrg8_probe_write", "mrg8_probe_sclk", "mrg8_probe_mad64", "mrg8_probe_xcc"}
     assert all(hasattr(probe.lib(), n) for n in names)
     # argument checks come before any device work
     assert probe.lib().mrg8_probe_write(None, 8, 0, 0, None) == -1
@@ -58,6 +58,8 @@ def test_probe_library_exports_its_header():
     assert probe.lib().mrg8_probe_sclk(None, None) == -1
     assert probe.lib().mrg8_probe_mad64(None, 4, 4, None) == -1
     assert probe.lib().mrg8_probe_mad64(ctypes.c_void_p(1024), 0, 4, None) == -1
+    assert probe.lib().mrg8_probe_xcc(None, 4, None) == -1
+    assert probe.lib().mrg8_probe_xcc(ctypes.c_void_p(1024), 0, None) == -1
 
 
 def test_library_is_gfx950_only():
