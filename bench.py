@@ -24,7 +24,8 @@ The same JSON line carries, measured in the same run (in this order):
                 each launch (the power manager's clamp), then the uint32 fill
   batched       4096 streams, log-uniform lengths in [2^10, 2^22), float64, one
                 launch with the autotuned shape; also as one replay of a
-                captured HIP graph (persistent grid, one tile per wave)
+                captured HIP graph (persistent grids taking the tiles in
+                address order or striding over them, one tile per wave)
   jumps         65536 streams jumped by uniform 63-bit steps, 256 uint32 each,
                 with the multiply-add rate against the measured VALU peak
   advance       2^20 device-resident states jumped by 200-bit (and 64-bit)
@@ -286,12 +287,16 @@ def bench_batched(args):
     log(f"batched: {total / (km * 1e-3) / 1e9:.1f} G samples/s, {gbs:.0f} GB/s, cfg {cfg}")
     out = {"streams": len(lengths), "outputs": total, "ms_per_launch": km, "gsps": total / (km * 1e-3) / 1e9,
            "gbs": gbs, "frac_of_peak": gbs / PEAK_GBS, "config": cfg,
-           "launch": "direct launch, " + ("one tile per wave" if cfg["blocks_per_cu"] == 0 else "persistent grid")}
+           "launch": "direct launch, " + ("one tile per wave" if cfg["blocks_per_cu"] == 0 else
+                                          "persistent grid taking the tiles in address order"
+                                          if cfg["blocks_per_cu"] < 0 else "persistent grid striding over the tiles")}
     # the configuration's named form: one replay of a captured HIP graph per
-    # run, with a persistent grid (best of a few shapes) and with one tile per wave
+    # run, with a persistent grid taking the tiles in address order, with one
+    # striding over them (best of a few shapes each) and with one tile per wave
     b.use_graph(True)
-    for label, shapes in (("persistent_graph", [(256, 4, 4096), (256, 8, 4096), (512, 2, 4096), (256, 4, 8192),
-                                                  (1024, 1, 4096), (256, 8, 2048)]),
+    for label, shapes in (("queue_graph", [(512, -2, 2048), (512, -3, 2048), (1024, -1, 2048), (256, -4, 2048)]),
+                          ("persistent_graph", [(256, 4, 4096), (256, 8, 4096), (512, 2, 4096), (256, 4, 8192),
+                                                (1024, 1, 4096), (256, 8, 2048)]),
                           ("tiles_graph", [(1024, 0, 2048)])):
         best = None
         for shape in shapes:

@@ -231,7 +231,11 @@ typedef struct mrg8_stream_desc {
 typedef struct mrg8_batch_config {
   int32_t block_threads;  /* 64..1024, multiple of 64 */
   int32_t blocks_per_cu;  /* 0: one tile per wave (grid from the tile count; best for long
-                             segments); > 0: persistent grid of CUs * blocks_per_cu blocks
+                             segments); < 0: persistent grid of CUs * -blocks_per_cu blocks
+                             that take groups of consecutive tiles in address order from a
+                             counter in the batch's device block (as fast or faster for long
+                             segments; runs of one batch are serialised, also across
+                             streams); > 0: persistent grid of CUs * blocks_per_cu blocks
                              (best for many short segments with long jumps) */
   uint64_t tile_outputs;  /* outputs per work item, multiple of 256, at most 2^40 */
 } mrg8_batch_config;

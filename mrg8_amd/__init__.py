@@ -256,6 +256,9 @@ class Batch:
         return {"block_threads": c.block_threads, "blocks_per_cu": c.blocks_per_cu, "tile_outputs": c.tile_outputs}
 
     def set_config(self, block_threads: int, blocks_per_cu: int, tile_outputs: int) -> None:
+        """blocks_per_cu 0: one tile per wave; < 0: a persistent grid of CUs x
+        -blocks_per_cu blocks taking the tiles in address order; > 0: a
+        persistent grid striding over the tiles.  The outputs are the same."""
         c = capi.BatchConfig(block_threads, blocks_per_cu, tile_outputs)
         capi.call("mrg8_batch_set_config", self._h, ctypes.byref(c))
 

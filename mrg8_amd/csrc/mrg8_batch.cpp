@@ -40,8 +40,10 @@ namespace mrg8 {
 int validate_batch_config(const mrg8_batch_config& c) {
   if (c.block_threads < 64 || c.block_threads > 1024 || c.block_threads % 64)
     return fail(MRG8_ERR_ARG, "block_threads %d must be a multiple of 64 in [64, 1024]", c.block_threads);
-  if (c.blocks_per_cu < 0 || c.blocks_per_cu * (c.block_threads / 64) > 32)
-    return fail(MRG8_ERR_ARG, "blocks_per_cu %d must be 0 (one tile per wave) or give at most 32 waves per CU",
+  if (c.blocks_per_cu < -32 || c.blocks_per_cu > 32 ||
+      (c.blocks_per_cu < 0 ? -c.blocks_per_cu : c.blocks_per_cu) * (c.block_threads / 64) > 32)
+    return fail(MRG8_ERR_ARG,
+                "blocks_per_cu %d must be 0 (one tile per wave) or give at most 32 waves per CU (either sign)",
                 c.blocks_per_cu);
   if (c.tile_outputs < 256 || c.tile_outputs % 256 || c.tile_outputs > kMaxTile)
     return fail(MRG8_ERR_ARG, "tile_outputs must be a positive multiple of 256, at most 2^40");
@@ -249,9 +251,10 @@ extern "C" int mrg8_batch_autotune(mrg8_batch_t b, void* stream, const char* cac
   }
   // threads-per-block x waves-per-CU x tile size, the counterpart of the
   // reference's manual TNUM x BS sweep (cuda/tests/test_mrg8_outer.cu:50-53)
-  // waves per CU 0: one tile per wave
+  // waves per CU 0: one tile per wave; > 0: persistent grid striding over
+  // the tiles; < 0: persistent grid taking the tiles in address order
   const int blocks[] = {256, 512, 1024};
-  const int waves_per_cu[] = {0, 4, 8, 16, 32};
+  const int waves_per_cu[] = {0, 4, 8, 16, 32, -16, -24};
   const uint64_t tiles[] = {2048, 4096, 8192, 16384};
   double best_ms = 1e30;
   mrg8_batch_config best_cfg = b->cfg;
@@ -261,7 +264,7 @@ extern "C" int mrg8_batch_autotune(mrg8_batch_t b, void* stream, const char* cac
       for (uint64_t t : tiles) {
         if (rc) break;
         const mrg8_batch_config c{bt, w / (bt / 64), t};
-        if ((w > 0 && c.blocks_per_cu < 1) || validate_batch_config(c) != MRG8_OK) continue;
+        if ((w != 0 && (c.blocks_per_cu == 0 || w % (bt / 64))) || validate_batch_config(c) != MRG8_OK) continue;
         const uint64_t nt = count_tiles(b->slot.segs(), b->slot.staged_count(), t);
         if (nt > b->max_tiles) continue;
         if (w == 0 && (nt + bt / 64 - 1) / (bt / 64) * (uint64_t)bt > 0xffffffffull) continue;  // 32-bit grid

@@ -625,6 +625,74 @@ __global__ __launch_bounds__(1024) void k_batch_tiles(BatchArgs a) {
   }
 }
 
+// Persistent batched kernel that takes its work in address order: each block
+// in turn takes the next group of (waves per block) consecutive tiles from a
+// counter in the batch block's header, and then works on it as k_batch_tiles
+// works on its block's tiles (waves 0-1 look up and jump the group's start
+// states and leave them in LDS, every wave generates one tile).  Blocks stay
+// resident, but the grid writes its tiles in the order a one-tile-per-wave
+// grid is dispatched in.  The last block to finish sets the counters back to
+// zero for the next launch; the counters make launches of one batch block
+// that overlap in time incorrect, which the host prevents.
+__global__ __launch_bounds__(1024) void k_batch_queue(BatchArgs a) {
+  __shared__ uint32_t start_state[16][8];
+  __shared__ uint64_t start_dst[16], start_cnt[16];
+  __shared__ int32_t start_fmt[16];
+  __shared__ uint32_t group;
+  const int lane = threadIdx.x & 63;
+  const BatchTables tab = batch_tables(a.block);
+  BatchHeader* hdr = reinterpret_cast<BatchHeader*>(const_cast<uint8_t*>(a.block));
+  const uint32_t wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const uint32_t nw = blockDim.x >> 6;
+  const uint64_t ngroups = (tab.ntiles + nw - 1) / nw;
+  for (;;) {
+    if (threadIdx.x == 0) group = atomicAdd(&hdr->queue_next, 1u);
+    __syncthreads();
+    const uint64_t g = group;  // the same value in every wave: the block leaves the loop together
+    if (g >= ngroups) break;
+    const uint64_t tile0 = g * nw;
+    if (wib * 8 < nw) {
+      const uint32_t target = wib * 8 + (lane >> 3);
+      const bool mine = target < nw && tile0 + target < tab.ntiles;
+      const TileInfo ti = lookup_tile(tab, a, mine ? tile0 + target : tile0);
+      uint32_t s[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) s[j] = ti.state[j];
+      jump_groups(s, mine ? ti.offset : 0, a.jump);
+      if (mine && (lane & 7) == 0) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) start_state[target][j] = s[j];
+        start_dst[target] = ti.dst;
+        start_cnt[target] = ti.cnt;
+        start_fmt[target] = ti.fmt;
+      }
+    }
+    __syncthreads();
+    if (tile0 + wib < tab.ntiles) {
+      uint32_t u[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) u[j] = __builtin_amdgcn_readfirstlane(start_state[wib][j]);
+      const uint64_t d = start_dst[wib], c = start_cnt[wib];
+      void* out = reinterpret_cast<void*>(readlane_u64(d, 0));
+      const uint64_t cnt = readlane_u64(c, 0);
+      switch (__builtin_amdgcn_readfirstlane(start_fmt[wib])) {
+        case 0: batch_tile_alone<0>(out, cnt, u, a.lane); break;
+        case 1: batch_tile_alone<1>(out, cnt, u, a.lane); break;
+        default: batch_tile_alone<2>(out, cnt, u, a.lane); break;
+      }
+    }
+    __syncthreads();  // LDS and `group` are rewritten by the next group
+  }
+  if (threadIdx.x == 0) {
+    __threadfence();  // this block's last take from the counter comes before its count of done blocks
+    if (atomicAdd(&hdr->queue_done, 1u) == gridDim.x - 1) {
+      // every block has left the loop: no block takes from the counter again
+      atomicExch(&hdr->queue_next, 0u);
+      atomicExch(&hdr->queue_done, 0u);
+    }
+  }
+}
+
 // ---- bulk advance: many states, offsets of up to kAdvanceWords words --------
 
 __device__ __forceinline__ const CoefRecord* wide_rec(const AdvanceArgs& a, int k, uint32_t d, int row) {
@@ -1020,6 +1088,12 @@ hipError_t launch_fill_mfma(const MfmaFillArgs& a, int format, hipStream_t strea
 
 hipError_t launch_batch(const BatchArgs& a, int grid, int block_threads, hipStream_t stream) {
   hipLaunchKernelGGL(k_batch, dim3(grid), dim3(block_threads), 0, stream, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_queue(const BatchArgs& a, int grid, int block_threads, hipStream_t stream) {
+  if (grid <= 0) return hipErrorInvalidConfiguration;
+  hipLaunchKernelGGL(k_batch_queue, dim3(grid), dim3(block_threads), 0, stream, a);
   return hipGetLastError();
 }
 

@@ -185,7 +185,12 @@ struct BatchHeader {
   uint64_t ntiles;
   uint64_t tile_begin_off;  // bytes from the block start
   uint64_t tile_seg_off;
-  uint64_t pad[4];
+  // Work counters of k_batch_queue: the next group of tiles to hand out and
+  // the blocks that have finished.  Zero when the block is staged; the last
+  // block of every launch sets them back to zero.
+  uint32_t queue_next;
+  uint32_t queue_done;
+  uint64_t pad[3];
 };
 static_assert(sizeof(BatchHeader) == 64, "header layout");
 
@@ -216,6 +221,10 @@ hipError_t launch_fill(const FillArgs& a, int format, int per_lane, int block_th
 // Blocks of a.waves_per_block waves.
 hipError_t launch_fill_mfma(const MfmaFillArgs& a, int format, hipStream_t stream);
 hipError_t launch_batch(const BatchArgs& a, int grid, int block_threads, hipStream_t stream);
+// Persistent grid of `grid` blocks taking groups of (waves per block)
+// consecutive tiles, in address order, from the counters in the batch
+// block's header.  Launches of one batch block must not overlap.
+hipError_t launch_batch_queue(const BatchArgs& a, int grid, int block_threads, hipStream_t stream);
 // One tile per wave: ceil(ntiles / waves per block) blocks.
 hipError_t launch_batch_tiles(const BatchArgs& a, uint64_t ntiles, int block_threads, hipStream_t stream);
 // Eight lanes per state; a.n * 8 must stay below 2^32.

@@ -94,8 +94,10 @@ extern "C" int mrg8_batch_generate_noalloc(int device, const mrg8_stream_desc* d
   a.jump = ctx->tables.jump;
   a.xcds = (uint32_t)ctx->num_xcc;
   a.xcd_group = xcd_run_blocks(*ctx, c.block_threads);
-  e = c.blocks_per_cu == 0 ? launch_batch_tiles(a, nt, c.block_threads, s)
-                           : launch_batch(a, ctx->num_cu * c.blocks_per_cu, c.block_threads, s);
+  // the staging copy above wrote the header's work counters as zero
+  e = c.blocks_per_cu == 0  ? launch_batch_tiles(a, nt, c.block_threads, s)
+      : c.blocks_per_cu < 0 ? launch_batch_queue(a, ctx->num_cu * -c.blocks_per_cu, c.block_threads, s)
+                            : launch_batch(a, ctx->num_cu * c.blocks_per_cu, c.block_threads, s);
   if (e != hipSuccess) return hip_fail(e, "batch launch (no-alloc)");
   return MRG8_OK;
 }

@@ -297,7 +297,8 @@ extern "C" int mrg8_server_create(int device, const mrg8_server_config* cfg, uin
   if (s->cfg.steps_in_flight > kMaxSteps) return fail(MRG8_ERR_ARG, "steps_in_flight must be at most %u", kMaxSteps);
   if (s->cfg.tile_outputs % 256 || s->cfg.tile_outputs > kMaxTile || s->cfg.block_threads % 64 ||
       s->cfg.block_threads > 1024 ||
-      s->cfg.blocks_per_cu * (s->cfg.block_threads / 64) > 32)
+      s->cfg.blocks_per_cu < -32 ||
+      (s->cfg.blocks_per_cu < 0 ? -s->cfg.blocks_per_cu : s->cfg.blocks_per_cu) * (s->cfg.block_threads / 64) > 32)
     return fail(MRG8_ERR_ARG, "invalid launch configuration");
   s->launch.block_threads = s->cfg.block_threads;
   s->launch.blocks_per_cu = s->cfg.blocks_per_cu;

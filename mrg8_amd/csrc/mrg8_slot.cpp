@@ -57,6 +57,7 @@ void BatchSlot::release() {
   copy_pending_ = run_pending_ = have_graph_ = false;
   last_stream_ = nullptr;
   last_marked_ = true;
+  last_queue_ = false;
   ctx_ = nullptr;
 }
 
@@ -92,6 +93,8 @@ size_t layout_batch_block(uint8_t* mem, uint32_t count, uint64_t tile, uint64_t 
   hdr->ntiles = t;
   hdr->tile_begin_off = tb_off;
   hdr->tile_seg_off = ts_off;
+  hdr->queue_next = 0;  // k_batch_queue's counters start at zero
+  hdr->queue_done = 0;
   *ntiles = t;
   return batch_block_bytes(count, std::min<uint64_t>(t, max_tiles));
 }
@@ -119,6 +122,8 @@ BatchArgs BatchSlot::args(const LaunchConfig& cfg) const {
 
 hipError_t BatchSlot::launch(const LaunchConfig& cfg, hipStream_t stream) const {
   if (cfg.blocks_per_cu == 0) return launch_batch_tiles(args(cfg), ntiles_, cfg.block_threads, stream);
+  if (cfg.blocks_per_cu < 0)
+    return launch_batch_queue(args(cfg), ctx_->num_cu * -cfg.blocks_per_cu, cfg.block_threads, stream);
   return launch_batch(args(cfg), ctx_->num_cu * cfg.blocks_per_cu, cfg.block_threads, stream);
 }
 
@@ -161,25 +166,33 @@ hipError_t BatchSlot::mark_last_run() {
 int BatchSlot::run(const LaunchConfig& cfg, hipStream_t stream, bool graph, bool track) {
   if (count_ == 0 || ntiles_ == 0) return MRG8_OK;
   if (ntiles_ > max_tiles_) return fail(MRG8_ERR_CAPACITY, "staged work needs more than max_tiles tiles");
+  const uint64_t wpb = (uint64_t)cfg.block_threads / 64;
   if (cfg.blocks_per_cu == 0) {  // one wave per tile: the grid (work-items in total) is a 32-bit quantity
-    const uint64_t wpb = (uint64_t)cfg.block_threads / 64;
     if ((ntiles_ + wpb - 1) / wpb * (uint64_t)cfg.block_threads > 0xffffffffull)
       return fail(MRG8_ERR_CAPACITY, "%llu tiles need more than 2^32 - 1 work-items at one tile per wave; use larger "
                   "tiles or a persistent grid", (unsigned long long)ntiles_);
+  } else if (cfg.blocks_per_cu < 0) {  // the work counter is 32 bits: groups plus one last take per block
+    if ((ntiles_ + wpb - 1) / wpb + (uint64_t)ctx_->num_cu * (uint64_t)-cfg.blocks_per_cu > 0xffffffffull)
+      return fail(MRG8_ERR_CAPACITY, "%llu tiles are more groups than the address-order grid counts; use larger tiles",
+                  (unsigned long long)ntiles_);
   }
   DeviceGuard dg(ctx_->device);
   int rc = graph ? capture(cfg) : MRG8_OK;
   if (rc) return rc;
   hipError_t e = hipSuccess;
+  // The device block may be overwritten once this slot's previous launch has
+  // finished, and an address-order grid (blocks_per_cu < 0) may start only
+  // after the previous one has reset the work counters in it: on the same
+  // stream both are ordered after the previous launch; on another stream
+  // they wait for it.  Runs next to an address-order run wait as well, so
+  // that two such runs are never apart by one unordered run in between.
+  if (run_pending_ && stream != last_stream_ && (dirty_ || cfg.blocks_per_cu < 0 || last_queue_)) {
+    if (mark_last_run() == hipSuccess) e = hipStreamWaitEvent(stream, run_done_, 0);
+    else if ((rc = wait_done()) != MRG8_OK) return rc;  // the last stream no longer accepts work
+    if (e != hipSuccess) return hip_fail(e, "hipStreamWaitEvent(previous run)");
+  }
   if (dirty_) {  // runs of an unchanged batch copy nothing
-    // The device block may be overwritten once this slot's previous launch
-    // has finished: on the same stream the copy is ordered after it; on
-    // another stream it waits for it.
-    if (run_pending_ && stream != last_stream_) {
-      if (mark_last_run() == hipSuccess) e = hipStreamWaitEvent(stream, run_done_, 0);
-      else if ((rc = wait_done()) != MRG8_OK) return rc;  // the last stream no longer accepts work
-    }
-    if (e == hipSuccess) e = launch_stage(hmem_, dmem_, used_bytes_, stream);
+    e = launch_stage(hmem_, dmem_, used_bytes_, stream);
     if (e == hipSuccess) e = hipEventRecord(staged_, stream);
     if (e != hipSuccess) return hip_fail(e, "batch staging copy");
     copy_pending_ = true;
@@ -190,6 +203,7 @@ int BatchSlot::run(const LaunchConfig& cfg, hipStream_t stream, bool graph, bool
   run_pending_ = true;
   last_stream_ = stream;
   last_marked_ = false;
+  last_queue_ = cfg.blocks_per_cu < 0;
   if (track && (e = mark_last_run()) != hipSuccess) return hip_fail(e, "hipEventRecord(run)");
   return MRG8_OK;
 }

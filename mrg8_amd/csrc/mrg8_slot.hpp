@@ -27,7 +27,8 @@ namespace mrg8 {
 
 struct LaunchConfig {
   int block_threads = 256;
-  int blocks_per_cu = 1;  // 0: one tile per wave (k_batch_tiles); > 0: persistent grid (k_batch)
+  int blocks_per_cu = 1;  // 0: one tile per wave (k_batch_tiles); > 0: persistent grid (k_batch);
+                          // < 0: persistent grid taking tiles in address order (k_batch_queue)
   uint64_t tile = 4096;
 };
 
@@ -93,6 +94,7 @@ class BatchSlot {
   bool copy_pending_ = false, run_pending_ = false, dirty_ = false;
   hipStream_t last_stream_ = nullptr;  // stream of the last run
   bool last_marked_ = true;            // run_done_ is recorded behind the last run
+  bool last_queue_ = false;            // the last run was an address-order grid (work counters)
   uint32_t count_ = 0;
   uint64_t ntiles_ = 0;
   hipGraph_t graph_ = nullptr;

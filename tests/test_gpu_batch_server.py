@@ -40,8 +40,9 @@ class Packed:
 
 
 @pytest.mark.parametrize("graph", [False, True], ids=["direct", "graph"])
-@pytest.mark.parametrize("shapes", [((256, 4, 4096), (512, 2, 16384)), ((1024, 0, 4096), (256, 0, 512))],
-                         ids=["persistent", "tile-per-wave"])
+@pytest.mark.parametrize("shapes", [((256, 4, 4096), (512, 2, 16384)), ((1024, 0, 4096), (256, 0, 512)),
+                                    ((512, -2, 2048), (256, -6, 4096))],
+                         ids=["persistent", "tile-per-wave", "address-order"])
 def test_batch_mixed_segments(gpu, oracle_mod, shapes, graph):
     rng = np.random.default_rng(21)
     n = 200
@@ -106,7 +107,53 @@ def test_batch_restaged_on_another_stream(gpu, oracle_mod, graph):
     b_s.destroy()
 
 
-@pytest.mark.parametrize("bpc", [1, 0], ids=["persistent", "tile-per-wave"])
+@pytest.mark.parametrize("graph", [False, True], ids=["direct", "graph"])
+def test_address_order_runs_on_several_streams(gpu, oracle_mod, graph):
+    """An address-order grid (blocks_per_cu < 0) takes its tiles from counters
+    in the batch's device block, which its last block sets back to zero: runs
+    of one unchanged batch issued on several streams without waiting, and a
+    tile-per-wave run between them, are serialised by the batch, so a later
+    run into a cleared buffer still writes every output."""
+    from mrg8_amd import hip
+
+    rng = np.random.default_rng(9)
+    n = 96
+    st = rng.integers(1, 2**31 - 1, size=(n, 8)).astype(np.uint32)
+    cnt = rng.integers(1000, 200000, size=n)
+    offs = rng.integers(0, 2**62, size=n, dtype=np.uint64)
+    total = int(cnt.sum())
+    starts = np.concatenate([[0], np.cumsum(cnt)[:-1]])
+    buf = hip.DeviceBuffer(total * 8)
+    streams = [hip.Stream() for _ in range(3)]
+    b = gpu.Batch(max_streams=n, max_tiles=1 << 16)
+    b.use_graph(graph)
+    b.set_config(256, -2, 2048)
+    b.set([(st[i], int(offs[i]), int(cnt[i]), buf.ptr + 8 * int(starts[i]), 0) for i in range(n)])
+    want = [oracle_mod.generate(oracle_mod.jump(st[i], int(offs[i])), int(cnt[i]), 0)[0] for i in range(0, n, 5)]
+    for rnd, order in enumerate(((0, 1, 2, 0), (2, 1, 1, 0))):
+        say("address-order streams", graph, rnd)
+        for k in order:
+            b.run(streams[k])
+        b.set_config(512, 0, 2048)  # a tile-per-wave run in between, on another stream
+        b.run(streams[(order[-1] + 1) % 3])
+        b.set_config(256, -2, 2048)
+        b.run(streams[order[-1]])
+        for s in streams:
+            s.synchronize()
+        buf.fill_byte(0)
+        b.run(streams[1])
+        streams[1].synchronize()
+        got = buf.copy_to_host(np.float64, total)
+        for j, i in enumerate(range(0, n, 5)):
+            part = got[int(starts[i]):int(starts[i]) + int(cnt[i])]
+            assert np.array_equal(part.view(np.uint64), want[j].view(np.uint64)), (rnd, i)
+    b.close()
+    buf.free()
+    for s in streams:
+        s.destroy()
+
+
+@pytest.mark.parametrize("bpc", [1, 0, -2], ids=["persistent", "tile-per-wave", "address-order"])
 def test_batch_tile_heads(gpu, oracle_mod, bpc):
     """Segments at many element offsets within a tile-alignment unit (the
     largest power of two dividing tile * element size), with counts around the
@@ -223,8 +270,9 @@ def test_batch_errors(gpu):
 
 
 @pytest.mark.parametrize("shape,graph", [(None, False), ((1024, 0, 4096), False), ((256, 4, 4096), True),
-                                         ((1024, 0, 2048), True)],
-                         ids=["default", "tile-per-wave", "persistent-graph", "tile-per-wave-graph"])
+                                         ((1024, 0, 2048), True), ((512, -2, 2048), False), ((512, -2, 2048), True)],
+                         ids=["default", "tile-per-wave", "persistent-graph", "tile-per-wave-graph", "address-order",
+                              "address-order-graph"])
 def test_c3_batched_full_size(gpu, oracle_mod, shape, graph):
     """4096 streams of log-uniform length in [2^10, 2^22), float64, one launch
     (a plain launch, or a replay of the captured graph, replayed twice with the
@@ -245,7 +293,7 @@ def test_c3_batched_full_size(gpu, oracle_mod, shape, graph):
     b.set([(states[i], 0, int(lengths[i]), buf.ptr + 8 * int(starts[i]), 0) for i in range(len(lengths))])
     want = oracle_mod.stream_wsums(states, np.zeros(len(lengths), dtype=np.uint64), lengths,
                                    np.zeros(len(lengths), dtype=np.int32))
-    for replay in range(2 if graph else 1):
+    for replay in range(2 if graph or (shape and shape[1] < 0) else 1):
         if replay:
             buf.fill_byte(0)  # the second replay must write every output again
         b.run()

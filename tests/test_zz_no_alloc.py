@@ -45,7 +45,8 @@ def descs(capi, states, offsets, counts, fmts, addrs):
     return arr
 
 
-@pytest.mark.parametrize("shape", [None, (256, 4, 4096), (512, 0, 256)], ids=["default", "persistent", "tiles-256"])
+@pytest.mark.parametrize("shape", [None, (256, 4, 4096), (512, 0, 256), (512, -2, 2048)],
+                         ids=["default", "persistent", "tiles-256", "address-order"])
 def test_noalloc_equals_batch_run(gpu, oracle_mod, shape):
     from mrg8_amd import capi, hip
 
