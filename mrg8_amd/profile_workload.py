@@ -16,8 +16,9 @@ last `--reps` dispatches of each kind):
      the matrix-core fill (outputs_per_lane 16, 1024-thread blocks, 2048 per
      wave)
   4. batched launch: 4096 streams, log-uniform lengths, float64, one
-     2048-output tile per wave, 1024-thread blocks in XCD runs of 32
-                                                                   (--batched)
+     2048-output tile per wave, 1024-thread blocks in XCD runs of 32; then
+     the same batch on a persistent grid of 512-thread blocks, 2 per CU,
+     taking 2048-output tiles in address order                     (--batched)
   5. jump-heavy batched launch: 65536 x (63-bit jump + 256 uint32) (--batched)
   6. bulk advance of 2^20 device-resident states by random 200-bit offsets,
      then by random 64-bit offsets                                 (--advance)
@@ -79,12 +80,15 @@ def main():
         out = hip.DeviceBuffer(total * 8)
         starts = np.concatenate([[0], np.cumsum(lengths)[:-1]]).astype(np.uint64)
         b = mrg8_amd.Batch(max_streams=len(lengths))
-        b.set_config(1024, 0, 2048)  # the default and the autotuner's pick for this batch on MI355X
         b.set([(states[i], 0, int(lengths[i]), out.ptr + 8 * int(starts[i]), mrg8_amd.F64)
                for i in range(len(lengths))])
-        for _ in range(args.warmup + args.reps):
-            b.run()
-        hip.synchronize()
+        # the batch default (one tile per wave), then the persistent grid that
+        # takes the tiles in address order
+        for shape in ((1024, 0, 2048), (512, -2, 2048)):
+            b.set_config(*shape)
+            for _ in range(args.warmup + args.reps):
+                b.run()
+            hip.synchronize()
         out.free()
         steps = workloads.jump_steps()
         m = len(steps)
