@@ -36,7 +36,8 @@ int mrg8_probe_sclk(uint64_t* d_out, void* stream);
 /* `blocks` blocks of one wave each; block b writes d_out[b] = the XCD it ran
  * on (bits [3:0] of the XCC_ID hardware register).  The fills and the
  * batched kernels map their blocks to the output in XCD runs on the
- * assumption that block b runs on XCD b mod (number of XCDs). */
+ * assumption that blocks are dealt to the XCDs in turn: block b on XCD
+ * (b + r) mod (number of XCDs), r depending on earlier launches. */
 int mrg8_probe_xcc(uint32_t* d_out, uint32_t blocks, void* stream);
 
 #ifdef __cplusplus

@@ -420,11 +420,13 @@ __device__ __forceinline__ void generate_run(typename Out<FMT>::T* __restrict__ 
 __device__ __forceinline__ uint64_t fill_first(const FillArgs& a, uint64_t w) { return w ? a.head + w * a.tile : 0; }
 
 // The part of the output that block b of an nb-block grid writes.  The
-// dispatcher deals blocks to the XCDs in turn (block b to XCD b mod x), so in
-// dispatch order the XCDs interleave block by block.  With group g > 1, each
-// XCD instead takes runs of g consecutive blocks' output within every span of
-// x g blocks.  Blocks past the last whole span keep their own part.  Either
-// way every part is written by exactly one block.
+// dispatcher deals blocks to the XCDs in turn (block b to XCD (b + r) mod x,
+// r where its turn stood after earlier launches), so in dispatch order the
+// XCDs interleave block by block.  With group g > 1, the blocks of one XCD
+// (b mod x alike) instead take runs of g consecutive blocks' output within
+// every span of x g blocks, whatever r is.  Blocks past the last whole span
+// keep their own part.  Either way every part is written by exactly one
+// block.
 __device__ __forceinline__ uint64_t xcd_block(uint32_t b, uint32_t nb, uint32_t x, uint32_t g) {
   if (g <= 1 || x == 0) return b;
   const uint32_t span = x * g, whole = nb / span * span;

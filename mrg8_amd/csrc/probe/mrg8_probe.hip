@@ -33,9 +33,10 @@ __global__ __launch_bounds__(512) void k_write_elem(double* __restrict__ d, uint
 }
 
 // Block b of nb writes the output of block xcd_block(b): blocks are dealt to
-// the x XCDs in turn, and with g > 1 each XCD takes runs of g consecutive
-// blocks' output within every span of x g blocks (blocks past the last whole
-// span keep their own).  The generator's fill maps its blocks the same way.
+// the x XCDs in turn (from wherever the turn stood), and with g > 1 the
+// blocks of one XCD take runs of g consecutive blocks' output within every
+// span of x g blocks (blocks past the last whole span keep their own).  The
+// generator's fill maps its blocks the same way.
 __device__ __forceinline__ uint64_t xcd_block(uint32_t b, uint32_t nb, uint32_t x, uint32_t g) {
   if (g <= 1 || x == 0) return b;
   const uint32_t span = x * g, whole = nb / span * span;

@@ -26,13 +26,17 @@ def test_write_probe_covers_buffer(gpu, mode, tile):
 
 
 def test_blocks_dealt_to_xcds_in_turn(gpu):
-    """Block b runs on XCD b mod X, X the number of XCDs the grid reaches.
-    The fills' outputs do not depend on it (xcd_block is a permutation of the
-    blocks either way); their speed does."""
+    """Block b runs on XCD (b + r) mod X, X the number of XCDs the grid
+    reaches and r where the dispatcher's turn stood after earlier launches
+    (block 0 need not start on XCD 0).  The XCD runs of the fills rely only
+    on the turn, not on r; the outputs rely on neither (xcd_block is a
+    permutation of the blocks)."""
     from mrg8_amd import probe
 
-    print("[case] xcc ids", flush=True)
-    ids = probe.xcc_ids(4096)
-    x = len(np.unique(ids))
-    assert x >= 1 and set(np.unique(ids)) == set(range(x))
-    assert np.array_equal(ids, np.arange(4096, dtype=np.uint32) % x), ids[:24]
+    for launch in range(3):  # successive launches may start the turn anywhere
+        print("[case] xcc ids, launch", launch, flush=True)
+        ids = probe.xcc_ids(4096 + 3 * launch)
+        x = len(np.unique(ids))
+        assert x >= 1 and set(np.unique(ids)) == set(range(x))
+        r = int(ids[0])
+        assert np.array_equal(ids, (np.arange(ids.size, dtype=np.uint32) + r) % x), ids[:24]
