@@ -128,9 +128,10 @@ hipError_t BatchSlot::launch(const LaunchConfig& cfg, hipStream_t stream) const 
 }
 
 int BatchSlot::capture(const LaunchConfig& cfg) {
-  // a one-tile-per-wave grid depends on the tile count, a persistent one does not
+  // a one-tile-per-wave grid depends on the tile count, a persistent one of
+  // either kind does not (its kernel reads the count from the device block)
   if (have_graph_ && cfg.block_threads == captured_.block_threads && cfg.blocks_per_cu == captured_.blocks_per_cu &&
-      cfg.tile == captured_.tile && (cfg.blocks_per_cu > 0 || captured_ntiles_ == ntiles_))
+      cfg.tile == captured_.tile && (cfg.blocks_per_cu != 0 || captured_ntiles_ == ntiles_))
     return MRG8_OK;
   if (int rc = wait_done()) return rc;  // the old graph may still be running
   if (exec_) (void)hipGraphExecDestroy(exec_);

@@ -72,10 +72,13 @@ def test_batch_mixed_segments(gpu, oracle_mod, shapes, graph):
         b.set_config(*shapes[1])  # a new shape, same outputs
 
 
+@pytest.mark.parametrize("shape", [None, (256, -2, 2048)], ids=["default", "address-order"])
 @pytest.mark.parametrize("graph", [False, True], ids=["direct", "graph"])
-def test_batch_restaged_on_another_stream(gpu, oracle_mod, graph):
+def test_batch_restaged_on_another_stream(gpu, oracle_mod, graph, shape):
     """A batch run on one stream, re-staged and run at once on another: the
-    new block's copy waits for the first run, and both runs' outputs hold."""
+    new block's copy waits for the first run, and both runs' outputs hold.
+    The two stagings have different tile counts: a persistent grid's graph
+    is reused for the second (its kernel reads the count from the block)."""
     from mrg8_amd import hip
 
     rng = np.random.default_rng(5)
@@ -87,8 +90,10 @@ def test_batch_restaged_on_another_stream(gpu, oracle_mod, graph):
     starts = [np.concatenate([[0], np.cumsum(cnt[r])[:-1]]) for r in range(2)]
     a_s, b_s = hip.Stream(), hip.Stream()
     b = gpu.Batch(max_streams=n, max_tiles=1 << 16)
+    if shape:
+        b.set_config(*shape)
     b.use_graph(graph)
-    say("restaged", graph)
+    say("restaged", graph, shape)
     for r, s in ((0, a_s), (1, b_s)):
         b.set([(st[r, i], int(offs[r, i]), int(cnt[r, i]), bufs[r].ptr + 8 * int(starts[r][i]), 0) for i in range(n)])
         b.run(s)
