@@ -235,7 +235,9 @@ typedef struct mrg8_batch_config {
                              that take groups of consecutive tiles in address order from a
                              counter in the batch's device block (as fast or faster for long
                              segments; runs of one batch are serialised, also across
-                             streams); > 0: persistent grid of CUs * blocks_per_cu blocks
+                             streams; the counter is 32 bits: a run with more groups of
+                             tiles than that is MRG8_ERR_CAPACITY); > 0: persistent grid
+                             of CUs * blocks_per_cu blocks
                              (best for many short segments with long jumps) */
   uint64_t tile_outputs;  /* outputs per work item, multiple of 256, at most 2^40 */
 } mrg8_batch_config;
