@@ -38,6 +38,7 @@ def xcc_ids(blocks: int = 4096) -> np.ndarray:
 
     buf = hip.DeviceBuffer(4 * blocks)
     try:
+        buf.fill_byte(0xFF)  # a block that wrote nothing leaves 0xffffffff
         rc = lib().mrg8_probe_xcc(ctypes.c_void_p(buf.ptr), blocks, None)
         if rc != 0:
             raise RuntimeError(f"mrg8_probe_xcc failed ({rc})")
