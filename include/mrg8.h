@@ -298,7 +298,10 @@ typedef struct mrg8_server_config {
   uint64_t tile_outputs;       /* outputs per work item, multiple of 256, at most 2^40
                                   (0 = automatic) */
   int32_t block_threads;       /* 0 = automatic */
-  int32_t blocks_per_cu;       /* 0 = automatic */
+  int32_t blocks_per_cu;       /* 0 = automatic (a persistent grid striding over the tiles, 1 block
+                                  per CU); as in mrg8_batch_config otherwise: < 0 takes the tiles in
+                                  address order (measured slower for the serving loop's short
+                                  request chunks) */
   uint32_t steps_in_flight;    /* launches in flight at once, 1..8 (0 = 3) */
   uint32_t age_limit_us;       /* requests admitted longer ago than this are served first,
                                   in admission order; the rest shortest-remaining first (0 = 1000) */

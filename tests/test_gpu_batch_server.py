@@ -370,17 +370,21 @@ def check_request(gpu, oracle_mod, c, base_state, fmt):
     return np.array_equal(host.view(np.uint8), want.view(np.uint8)), host
 
 
-def test_server_requests_match_oracle(gpu, oracle_mod):
+@pytest.mark.parametrize("grid", [{}, {"block_threads": 256, "blocks_per_cu": -2}], ids=["default", "address-order"])
+def test_server_requests_match_oracle(gpu, oracle_mod, grid):
+    """Every request's outputs equal the oracle's, with the serving loop's
+    default persistent grid and with the address-order grid (its steps run on
+    several streams, each on a batch block of its own)."""
     rng = np.random.default_rng(31)
     srv = gpu.Server(num_streams=64, max_output_bytes=64 << 20, base_seed=1000, step_outputs=1 << 20,
-                     tile_outputs=8192)
+                     tile_outputs=8192, **grid)
     reqs = {}
     for k in range(300):
         sid = int(rng.integers(0, 64))
         cnt = int(rng.integers(1, 200000))
         fmt = int(rng.integers(0, 3))
         reqs[srv.submit(sid, cnt, fmt)] = (sid, fmt)
-    say("server-oracle")
+    say("server-oracle", grid)
 
     def check(c):
         sid, fmt = reqs[c["request_id"]]
